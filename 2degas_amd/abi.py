@@ -1,0 +1,95 @@
+"""ctypes mirror of the C-ABI structs in include/qmc_hip.h.
+
+Only plumbing: the structs are filled from a :class:`Deck` and handed to the
+HIP library (and, in tests, to the CPU oracle, which shares the layout).
+"""
+import ctypes as C
+
+import numpy as np
+
+c_double_p = C.POINTER(C.c_double)
+c_int32_p = C.POINTER(C.c_int32)
+
+QMC_OK = 0
+QMC_ERR_ARG = 1
+QMC_ERR_LIMIT = 2
+QMC_ERR_SINGULAR = 3
+QMC_ERR_HIP = 4
+QMC_ERR_NOTWIST = 5
+QMC_ERR_STATE = 6
+
+QMC_PREC_FP64 = 0
+QMC_PREC_MIXED = 1
+
+
+class QmcTables(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int32),
+        ("ntypes", C.c_int32),
+        ("np", C.c_int32 * 2),
+        ("hbs2m", C.c_double * 2),
+        ("el", C.c_double * 2),
+        ("drt", C.c_double),
+        ("ntab", C.c_int32),
+        ("ldut", C.c_int32),
+        ("ut", c_double_p),
+        ("ngrid", c_int32_p),
+        ("tail", c_double_p),
+        ("iexp", c_int32_p),
+        ("nk_ewald", c_int32_p),
+        ("iu2", C.c_int32 * 4),
+        ("iu3", C.c_int32 * 4),
+        ("iub", C.c_int32 * 4),
+        ("iv2", C.c_int32 * 4),
+        ("v2value", C.c_double * 4),
+        ("v0", C.c_double),
+        ("nk", C.c_int32),
+        ("kvec", c_double_p),
+        ("irhok", C.c_int32 * 2),
+        ("nrhok", C.c_int32),
+        ("ntheta", C.c_int32),
+        ("adrift", C.c_double),
+        ("delta", C.c_double),
+    ]
+
+
+class QmcInfo(C.Structure):
+    _fields_ = [
+        ("nptot", C.c_int32),
+        ("npnorm", C.c_int32),
+        ("n_scalar_props", C.c_int32),
+        ("n_props", C.c_int32),
+        ("jetot", C.c_int32),
+        ("jltf", C.c_int32),
+        ("jacc", C.c_int32),
+        ("jpot0", C.c_int32),
+        ("jkin0", C.c_int32),
+        ("je2", C.c_int32),
+        ("jgg", C.c_int32),
+        ("jefp", C.c_int32),
+        ("jrhok", C.c_int32),
+        ("jpot", C.c_int32 * 2),
+        ("jkin", C.c_int32 * 2),
+        ("jun", C.c_int32 * 2),
+        ("ng", C.c_int32),
+        ("has_det", C.c_int32),
+    ]
+
+    def as_dict(self):
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if hasattr(v, "__len__") else int(v)
+        return d
+
+
+def dptr(a):
+    """Pointer to a C-contiguous float64 numpy array."""
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(c_double_p)
+
+
+def iptr(a):
+    """Pointer to a C-contiguous int32 numpy array."""
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(c_int32_p)

@@ -1,0 +1,165 @@
+/*
+ * qmc_hip.h -- C-ABI of the MI355X walker-evaluation library (lib2degas_hip.so).
+ *
+ * Plain C: scalars by value, arrays by pointer, int return codes (0 = ok),
+ * no allocation visible to the caller, no torch or HIP types.  Every entry
+ * point replaces one seam of the reference Fortran (fabioaffinito/2degas);
+ * the citation in front of each declaration names it.  The Fortran side binds
+ * these through ISO_C_BINDING (2degas_amd/fortran/qmc_hip_mod.f90).
+ *
+ * Units and layouts follow the reference: energies in Rydberg, lengths in
+ * units of rs, positions x[walker][particle][2] with all particles of type 1
+ * first (the reference's x_new(2,mnp) column-major == C [particle][2]).
+ */
+#ifndef QMC_HIP_H
+#define QMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+enum {
+  QMC_OK = 0,
+  QMC_ERR_ARG = 1,         /* bad argument / unsupported deck feature       */
+  QMC_ERR_LIMIT = 2,       /* size above a compiled limit                   */
+  QMC_ERR_SINGULAR = 3,    /* orbital matrix singular (reptation.f:10318)   */
+  QMC_ERR_HIP = 4,         /* HIP runtime error                             */
+  QMC_ERR_NOTWIST = 5,     /* determinant requested but no g-vectors set    */
+  QMC_ERR_STATE = 6        /* call order (e.g. sweep before init)           */
+};
+
+/* Compiled limits of the device path. */
+#define QMC_MAX_PARTICLES 256  /* total particles per walker                */
+#define QMC_MAX_ORBITALS 64    /* particles per spin in one determinant      */
+#define QMC_MAX_TYPES 2
+
+/*
+ * Deck tables, copied once into the device context.  Mirrors the reference
+ * COMMON blocks filled by `input` (reptation.f:12-710):
+ *   /c_sys/ np, hbs2m, ntypes (ewald.h:124-130), /c_box/ el (ewald.h:150-151),
+ *   /c_tables/ ut, tail, iexp, iu2table, iu3table, iubtable, iv2table
+ *   (ewald.h:101-116), /c_dist/ drt, ngrid (ewald.h:84-99),
+ *   /c_kspace/ kvec, nk (ewald.h:160-164), /c_ewald/ nk_ewald (ewald.h:272),
+ *   v0, adrift (ewald.h:175-179), ntheta (ewald.h:288-290),
+ *   /c_rhok/ irhok, nrhok (ewald.h:72-76).
+ * `ut` keeps the reference layout ut(0:ldut-1, 4, ntab), column-major:
+ * coefficient c (0..3) of row i of table t (0-based) is ut[(t*4+c)*ldut+i].
+ * Rows ngrid+1..ngrid+nk_ewald of column 0 hold vk and row ngrid+nk_ewald+1
+ * holds vk0 for a table read with a `kspace` block (reptation.f:4716-4724).
+ * Table selectors are 1-based table numbers (0 = none), indexed [it*2+jt].
+ */
+typedef struct qmc_tables {
+  int32_t ndim;            /* must be 2                                    */
+  int32_t ntypes;          /* 1 or 2                                       */
+  int32_t np[2];           /* particles per type                           */
+  double hbs2m[2];         /* hbar^2/2m per type (reference units)         */
+  double el[2];            /* box sides                                    */
+  double drt;              /* radial grid step shared by all tables        */
+  int32_t ntab;            /* number of tables                             */
+  int32_t ldut;            /* leading dimension of ut                      */
+  const double* ut;        /* [ntab][4][ldut]                              */
+  const int32_t* ngrid;    /* [ntab]                                       */
+  const double* tail;      /* [ntab]                                       */
+  const int32_t* iexp;     /* [ntab]                                       */
+  const int32_t* nk_ewald; /* [ntab]                                       */
+  int32_t iu2[4], iu3[4], iub[4], iv2[4];
+  double v2value[4];       /* pair-potential multipliers (reference: 1.0)  */
+  double v0;               /* constant potential (`v0` keyword)            */
+  int32_t nk;              /* k-vectors read by `kspace`                   */
+  const double* kvec;      /* [nk][2]                                      */
+  int32_t irhok[2];        /* rhok slot of each type, 0 = not requested    */
+  int32_t nrhok;
+  int32_t ntheta;          /* twists per direction (0 = Gamma, no efp)     */
+  double adrift;           /* drift cutoff (0 = off, reference default)    */
+  double delta;            /* time step used by the drift cutoff           */
+} qmc_tables;
+
+/* Sizes of a context (slot numbers are 1-based like the reference's). */
+typedef struct qmc_info {
+  int32_t nptot;           /* total particles                              */
+  int32_t npnorm;          /* moving particles                             */
+  int32_t n_scalar_props;  /* scalar property slots (reptation.f:577-616)  */
+  int32_t n_props;         /* scalars + 2*nk per rhok type                 */
+  int32_t jetot, jltf, jacc, jpot0, jkin0, je2, jgg, jefp, jrhok;
+  int32_t jpot[2], jkin[2], jun[2];
+  int32_t ng;              /* g-vectors currently set                       */
+  int32_t has_det;         /* backflow determinant active                  */
+} qmc_info;
+
+typedef struct qmc_ctx qmc_ctx;
+
+/* Precision of an evaluation. */
+enum { QMC_PREC_FP64 = 0, QMC_PREC_MIXED = 1 };
+
+/*
+ * Create a device context from deck tables on HIP device `device`.
+ * Replaces the state set up by `input` (reptation.f:12-710) that
+ * compute_properties reads.  Validates the deck: every pair of types with
+ * particles must carry the same table kinds (the reference's pair counters
+ * assume it, reptation.f:2993-3038).
+ */
+int qmc_ctx_create(const qmc_tables* tables, int device, qmc_ctx** out);
+int qmc_ctx_destroy(qmc_ctx* ctx);
+int qmc_ctx_info(const qmc_ctx* ctx, qmc_info* out);
+
+/*
+ * Set the g-vectors of the determinant orbitals (first np(it) are occupied
+ * per spin).  Replaces the gvec/gtens COMMON written by `input`
+ * (reptation.f:674-708) or by `tabc`/`twistg` (reptation.f:9559-9638,
+ * 9818-9861).  gvec is [ng][2].
+ */
+int qmc_set_twist(qmc_ctx* ctx, const double* gvec, int ng);
+
+/*
+ * Host helper: the twisted, |g|^2-sorted g-vector list exactly as
+ * twistg(ith,jth) builds it, including the reference's heap sort SORTTF
+ * (reptation.f:9514-9557, 9818-9861).  With ntheta == 0 it returns the
+ * untwisted list of `input` (reptation.f:674-708).  gvec_out holds 2*nk+1
+ * vectors.
+ */
+int qmc_twist_gvec(const double* kvec, int nk, const double* el, int ntheta,
+                   int ith, int jth, double* gvec_out);
+
+/*
+ * Batched local-energy evaluation: for each of nw walkers, everything
+ * compute_properties(1) (reptation.f:2661-2727) writes:
+ *   props[w][n_props]  p_new(1..n_props) (slot jacc is left as passed in),
+ *   g[w][nptot][2]     g_new  = grad(-log psi),
+ *   h[w][2]            h_new  = per-type laplacian of -log psi,
+ *   grad2_ph[w]        fixed-phase |Im grad log D|^2 sum (in/out: left as
+ *                      passed when the determinant underflow guard of
+ *                      reptation.f:10087-10092 fires, as the reference does).
+ * Host pointers; the library stages them through device memory.
+ * status[w] (optional, may be NULL) gets 1 where the underflow guard fired.
+ */
+int qmc_eval_batch(qmc_ctx* ctx, int nw, const double* x, double* props,
+                   double* g, double* h, double* grad2_ph, int32_t* status);
+
+/*
+ * Same as qmc_eval_batch with all arrays already in device memory and the
+ * work queued on `stream` (a hipStream_t, NULL = default stream); returns
+ * once queued.  `precision` selects QMC_PREC_FP64 or QMC_PREC_MIXED.
+ */
+int qmc_eval_batch_device(qmc_ctx* ctx, int nw, const double* d_x,
+                          double* d_props, double* d_g, double* d_h,
+                          double* d_grad2_ph, int32_t* d_status, int precision,
+                          void* stream);
+
+/* Human-readable text of an error code, and of the last HIP error seen. */
+const char* qmc_strerror(int code);
+const char* qmc_last_error(void);
+
+/*
+ * Algorithmic floating-point work of one local-energy evaluation for a
+ * system of nup+ndown particles with nk_ewald k-vectors (real flops, FMA=2):
+ * the per-evaluation count used for roofline fractions.
+ */
+double qmc_flops_per_eval(int nup, int ndown, int nk_ewald);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QMC_HIP_H */

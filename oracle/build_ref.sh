@@ -1,0 +1,53 @@
+#!/bin/bash
+# Builds the reference Fortran path into oracle/_ref/ (test infrastructure).
+#
+# Sources are read where they lie (REF, default /root/reference); nothing
+# from them is copied into the repository.  The compile-time limits of the
+# reference's ewald.h are raised for the larger decks (morbit, mnk, mgrid,
+# m_props_in_stack, mstack); that patched header and the copy of
+# reptation.f it has to sit beside live only in a temporary directory that
+# is removed on exit.  Outputs (binaries only) go to oracle/_ref/:
+#   refeval   fixed-configuration / move-replay / timing harness (refeval.f)
+#   refqmc    the reference's own driver entry (refmain.f)
+# Compiler: amdflang (ROCm LLVM Flang), or gfortran when present.
+set -euo pipefail
+REF=${REF:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+OPT=${OPT:--O3}
+if [ ! -f "$REF/reptation.f" ]; then
+  echo "build_ref: reference sources not found at $REF; keeping existing oracle/_ref" >&2
+  exit 0
+fi
+FC=${FC:-}
+if [ -z "$FC" ]; then
+  if command -v gfortran >/dev/null 2>&1; then FC=gfortran
+  elif command -v amdflang >/dev/null 2>&1; then FC=amdflang
+  elif [ -x /opt/rocm/bin/amdflang ]; then FC=/opt/rocm/bin/amdflang
+  else echo "build_ref: no Fortran compiler" >&2; exit 1; fi
+fi
+TMP=$(mktemp -d /tmp/degas_ref.XXXXXX)
+trap 'rm -rf "$TMP"' EXIT
+cp "$REF/reptation.f" "$REF/fake_mpi/mpif.f" "$REF/fake_mpi/mpif.h" "$TMP/"
+# raised limits: N<=166, 64 orbitals/spin, 2*nk+1<=2048 g-vectors,
+# ngrid+nk_ewald+1<=2100 table rows, n_props<=4000 stack slots
+sed -e 's/mgrid= 901 ,/mgrid=2100 ,/' \
+    -e 's/morbit=  42,mnk=196/morbit=  64,mnk=2048/' \
+    -e 's/mstack=10000,/mstack=1000,/' \
+    -e 's/m_props_in_stack=900,/m_props_in_stack=4000,/' \
+    "$REF/ewald.h" > "$TMP/ewald.h"
+for pat in 'mgrid=2100' 'morbit=  64,mnk=2048' 'mstack=1000' 'm_props_in_stack=4000'; do
+  grep -q "$pat" "$TMP/ewald.h" || { echo "build_ref: limit patch '$pat' did not apply" >&2; exit 1; }
+done
+cp "$HERE/refeval.f" "$HERE/refmain.f" "$TMP/"
+cd "$TMP"
+FFLAGS="$OPT -w"
+$FC $FFLAGS -c reptation.f -o reptation.o 2> build.log || { cat build.log >&2; exit 1; }
+$FC $FFLAGS -c mpif.f -o mpif.o
+$FC $FFLAGS -c refeval.f -o refeval.o
+$FC $FFLAGS -c refmain.f -o refmain.o
+mkdir -p "$OUT"
+$FC $FFLAGS refeval.o reptation.o mpif.o -o "$OUT/refeval"
+$FC $FFLAGS refmain.o reptation.o mpif.o -o "$OUT/refqmc"
+echo "$FC $($FC --version 2>/dev/null | head -1) $OPT" > "$OUT/COMPILER"
+echo "build_ref: built $OUT/refeval $OUT/refqmc with $FC $OPT"

@@ -1,0 +1,105 @@
+      program refeval
+c Test-only harness around the reference Fortran (never shipped).
+c It calls the reference's own routines and dumps their outputs so
+c that the C restatement and the HIP library can be checked against
+c them.  Linked with reptation.f + fake_mpi/mpif.f by build_ref.sh.
+c
+c Control file refeval.ctl (list-directed), in the deck directory:
+c   mode                 1 = fixed-configuration evaluation
+c                        2 = VMC move/accept replay
+c                        3 = timing: mode 1 without output, repeated
+c                            ncfg times over the first config block
+c   ith jth              twist indices (used when the deck sets ntwists)
+c   ncfg                 configurations (mode 1) or moves (mode 2)
+c   delta                time step of the move (mode 2)
+c   cfgfile              raw float64 positions, [ncfg][nptot][2]
+c   outfile              raw output stream (layout below)
+c
+c Mode 1 follows read_conf (reptation.f:2452-2502) without the stack:
+c x_new <- config; compute_properties(1) (reptation.f:2661).
+c Mode 2 follows vmc (reptation.f:973-1017): one initial
+c compute_properties + metropolis_test(1) as read_conf does, then
+c move (reptation.f:4617) + metropolis_test (reptation.f:2622).
+      implicit none
+      include 'ewald.h'
+      integer mode,ncfg,icfg,ip,idim,ith0,jth0,i,it,k,seedv(4)
+      integer istp,nk_out,nbuf
+      integer*8 nrate,t0,t1
+      parameter(nbuf=64)
+      real*8 p,del,xb(2,mnp,nbuf)
+      character*256 cfgfile,outfile
+      mytid=0
+      nproc=1
+      call input
+      open(11,file='refeval.ctl',status='old')
+      read(11,*) mode
+      read(11,*) ith0,jth0
+      read(11,*) ncfg
+      read(11,*) del
+      read(11,'(a)') cfgfile
+      read(11,'(a)') outfile
+      close(11)
+      if(ntheta.ne.0)call twistg(ith0,jth0)
+      nk_out=0
+      if(nrhok.ne.0)nk_out=nk
+      open(12,file=cfgfile,access='stream',form='unformatted'
+     &    ,status='old')
+      open(13,file=outfile,access='stream',form='unformatted'
+     &    ,status='replace')
+      write(13) n_scalar_props,nptot,ntypes,nk_out,nrhok
+      if(mode.eq.1)then
+       do icfg=1,ncfg
+        read(12) ((x_new(idim,ip),idim=1,2),ip=1,nptot)
+        call compute_properties(1)
+        write(13) (p_new(i),i=1,n_scalar_props)
+        write(13) ((g_new(idim,ip),idim=1,2),ip=1,nptot)
+        write(13) (h_new(it),it=1,ntypes)
+        write(13) grad2_ph
+        do it=1,nrhok
+         write(13) (rhok(k,it),k=1,2*nk)
+        enddo
+       enddo
+      elseif(mode.eq.2)then
+       delta=del
+       call sigma
+       mage=1.d30
+       read(12) ((x_new(idim,ip),idim=1,2),ip=1,nptot)
+       call compute_properties(1)
+       p=1.d0
+       call metropolis_test(p)
+       do istp=1,ncfg
+        call savern(seedv)
+c per move: RNG state and walker before the move, then p, age after
+c the test, proposed -log psi and local energy, kept -log psi, energy
+        write(13) seedv
+        write(13) ((x_old(idim,ip),idim=1,2),ip=1,nptot)
+        call move(p)
+        call metropolis_test(p)
+        write(13) p,age,p_new(jltf),p_new(jetot),p_old(jltf)
+     &           ,p_old(jetot)
+       enddo
+       write(13) ((x_old(idim,ip),idim=1,2),ip=1,nptot)
+       call savern(seedv)
+       write(13) seedv
+      elseif(mode.eq.3)then
+       do icfg=1,nbuf
+        read(12) ((xb(idim,ip,icfg),idim=1,2),ip=1,nptot)
+       enddo
+       call system_clock(t0,nrate)
+       do istp=1,ncfg
+        icfg=mod(istp-1,nbuf)+1
+        do ip=1,nptot
+         x_new(1,ip)=xb(1,ip,icfg)
+         x_new(2,ip)=xb(2,ip,icfg)
+        enddo
+        call compute_properties(1)
+       enddo
+       call system_clock(t1,nrate)
+       write(6,'(a,i10,a,es16.8)') 'REFEVAL_TIMING evals ',ncfg
+     &      ,' seconds ',dble(t1-t0)/dble(nrate)
+       write(13) dble(t1-t0)/dble(nrate),p_new(jetot)
+      endif
+      close(12)
+      close(13)
+      stop
+      end
