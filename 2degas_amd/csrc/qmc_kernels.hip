@@ -1,0 +1,820 @@
+// qmc_kernels.hip -- FP64 walker-evaluation kernels for gfx950.
+//
+// One local-energy evaluation (compute_properties(1), reptation.f:2661-2727)
+// of a batch of walkers is four launches:
+//   k_pairs   one workgroup per walker: pair geometry, u2 and u3 Jastrow
+//             (two_body/three_body/dotg, reptation.f:2777-3046), backflow
+//             quasi-coordinates (reptation.f:10023-10058), real-space Ewald
+//             (reptation.f:3994-4033), rho_k and k-space Ewald
+//             (reptation.f:4224-4243, 4034-4052).
+//   k_det     one workgroup per (walker, spin): plane-wave orbital matrix of
+//             the quasi-coordinates, LU with LINPACK pivoting and inverse in
+//             LDS (zgefa/zgedi, reptation.f:9227-9472), F = A^-1 dA, drift
+//             and the diagonal Laplacian terms (reptation.f:10159-10240).
+//   k_cross   one workgroup per (walker, spin): the backflow Laplacian cross
+//             term sum_ip tr(C_ip^2) (reptation.f:10241-10259) as a
+//             symmetric contraction of S = V V^T against M = Re(F o F^T).
+//   k_final   kinetic and potential energies and the property slots
+//             (kinetic, reptation.f:2729-2775; potential tail, 4088-4096).
+#include "qmc_dev.h"
+
+// ---------------------------------------------------------------------------
+// k_pairs
+// LDS: xs[2N], Gs[2N], red[8]
+__global__ __launch_bounds__(QMC_BLOCK) void k_pairs(DevSys s, WsLayout L,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ ws,
+                                                     double* __restrict__ props, int tpp_log2) {
+  extern __shared__ double sm[];
+  const int w = blockIdx.x;
+  const int N = s.nptot;
+  double* xs = sm;
+  double* Gs = sm + 2 * N;
+  double* red = sm + 4 * N;
+  const double* xw = x + (int64_t)w * 2 * N;
+  for (int k = threadIdx.x; k < 2 * N; k += QMC_BLOCK) xs[k] = xw[k];
+  __syncthreads();
+  const int tpp = 1 << tpp_log2;
+  const int lane = threadIdx.x & (tpp - 1);
+  const int grp = threadIdx.x >> tpp_log2;
+  const int ngrp = QMC_BLOCK >> tpp_log2;
+  double* wsw = ws + (int64_t)w * L.per_walker;
+  double acc_ltf = 0.0, acc_h0 = 0.0, acc_h1 = 0.0, acc_v0 = 0.0, acc_v1 = 0.0;
+
+  // pass 1: everything that needs one sweep over partners
+  for (int i0 = 0; i0 < N; i0 += ngrp) {
+    const int i = i0 + grp;
+    const bool act = i < N;
+    double gx = 0, gy = 0, Gx = 0, Gy = 0, qx = 0, qy = 0, ltf = 0, hh = 0, vv = 0;
+    const int ti = act ? ptype(s, i) : 0;
+    if (act) {
+      const double xi = xs[2 * i], yi = xs[2 * i + 1];
+      for (int j = lane; j < N; j += tpp) {
+        if (j == i) continue;
+        const int tp = tpair(ti, ptype(s, j));
+        PairGeo p = pair_geo(s, xi, yi, xs[2 * j], xs[2 * j + 1]);
+        double f, df, ddf;
+        if (s.has_u2) {
+          spline3(s, s.tu2[tp], p.ind, p.rem, f, df, ddf);
+          ltf += 0.5 * f;
+          const double dtr = df * p.byr;
+          gx += dtr * p.ux;
+          gy += dtr * p.uy;
+          hh += ddf + dtr;
+        }
+        if (s.has_u3) {
+          spline3(s, s.tu3[tp], p.ind, p.rem, f, df, ddf);
+          Gx -= f * p.ux;
+          Gy -= f * p.uy;
+          // (xi r)^2 pair term of the three-body factor (reptation.f:2822-2832)
+          const double r = p.r;
+          const double tr = f * r;
+          const double d3 = 2.0 * f * (f + r * df);
+          const double dd3 = 2.0 * (f * f + 4.0 * r * f * df + r * r * (df * df + ddf * f));
+          ltf += 0.5 * tr * tr;
+          gx += d3 * p.ux;
+          gy += d3 * p.uy;
+          hh += dd3 + d3;
+        }
+        if (s.has_b) {
+          spline3(s, s.tub[tp], p.ind, p.rem, f, df, ddf);
+          qx += f * p.ux;
+          qy += f * p.uy;
+        }
+        if (s.has_v2) {
+          vv += 0.5 * spline0(s, s.tv2[tp], p.ind, p.rem, p.byr) * s.v2value[tp];
+        }
+      }
+    }
+    for (int o = 1; o < tpp; o <<= 1) {
+      gx += __shfl_xor(gx, o, 64);
+      gy += __shfl_xor(gy, o, 64);
+      Gx += __shfl_xor(Gx, o, 64);
+      Gy += __shfl_xor(Gy, o, 64);
+      qx += __shfl_xor(qx, o, 64);
+      qy += __shfl_xor(qy, o, 64);
+      ltf += __shfl_xor(ltf, o, 64);
+      hh += __shfl_xor(hh, o, 64);
+      vv += __shfl_xor(vv, o, 64);
+    }
+    if (act && lane == 0) {
+      Gs[2 * i] = Gx;
+      Gs[2 * i + 1] = Gy;
+      wsw[L.off_gj + 2 * i] = gx;
+      wsw[L.off_gj + 2 * i + 1] = gy;
+      wsw[L.off_q + 2 * i] = xs[2 * i] + qx;
+      wsw[L.off_q + 2 * i + 1] = xs[2 * i + 1] + qy;
+      acc_ltf += ltf - 0.5 * (Gx * Gx + Gy * Gy);
+      if (ti == 0) { acc_h0 += hh; acc_v0 += vv; }
+      else { acc_h1 += hh; acc_v1 += vv; }
+    }
+  }
+  __syncthreads();
+
+  // pass 2: three-body contraction (dotg, reptation.f:2928-2979), written per
+  // particle i as g_i += sum_k M_ik (G_i - G_k),
+  // h_i += -|sum_k M_ik|^2 - sum_k |M_ik|^2 + sum_k a_ik u_ik.(G_i - G_k)
+  // with M = xi I + (xi'/r) u u^T and a = xi'' + 3 xi'/r.
+  if (s.has_u3) {
+    for (int i0 = 0; i0 < N; i0 += ngrp) {
+      const int i = i0 + grp;
+      const bool act = i < N;
+      double g3x = 0, g3y = 0, e11 = 0, e12 = 0, e22 = 0, nm2 = 0, h3 = 0;
+      const int ti = act ? ptype(s, i) : 0;
+      if (act) {
+        const double xi = xs[2 * i], yi = xs[2 * i + 1];
+        const double Gix = Gs[2 * i], Giy = Gs[2 * i + 1];
+        for (int j = lane; j < N; j += tpp) {
+          if (j == i) continue;
+          const int tp = tpair(ti, ptype(s, j));
+          PairGeo p = pair_geo(s, xi, yi, xs[2 * j], xs[2 * j + 1]);
+          double t, dt, ddt;
+          spline3(s, s.tu3[tp], p.ind, p.rem, t, dt, ddt);
+          const double c = dt * p.byr;
+          const double m11 = t + c * p.ux * p.ux, m12 = c * p.ux * p.uy, m22 = t + c * p.uy * p.uy;
+          const double dGx = Gix - Gs[2 * j], dGy = Giy - Gs[2 * j + 1];
+          g3x += m11 * dGx + m12 * dGy;
+          g3y += m12 * dGx + m22 * dGy;
+          e11 += m11;
+          e12 += m12;
+          e22 += m22;
+          nm2 += m11 * m11 + 2.0 * m12 * m12 + m22 * m22;
+          h3 += (ddt + 3.0 * c) * (p.ux * dGx + p.uy * dGy);
+        }
+      }
+      for (int o = 1; o < tpp; o <<= 1) {
+        g3x += __shfl_xor(g3x, o, 64);
+        g3y += __shfl_xor(g3y, o, 64);
+        e11 += __shfl_xor(e11, o, 64);
+        e12 += __shfl_xor(e12, o, 64);
+        e22 += __shfl_xor(e22, o, 64);
+        nm2 += __shfl_xor(nm2, o, 64);
+        h3 += __shfl_xor(h3, o, 64);
+      }
+      if (act && lane == 0) {
+        h3 = h3 - (e11 * e11 + 2.0 * e12 * e12 + e22 * e22) - nm2;
+        wsw[L.off_gj + 2 * i] += g3x;
+        wsw[L.off_gj + 2 * i + 1] += g3y;
+        if (ti == 0) acc_h0 += h3;
+        else acc_h1 += h3;
+      }
+    }
+  }
+
+  // rho_k per type and the k-space Ewald sum
+  double pk0 = 0.0, pk1 = 0.0, pkt = 0.0;
+  if (s.nrhok) {
+    double* pw = props + (int64_t)w * s.n_props;
+    for (int k = threadIdx.x; k < s.nk; k += QMC_BLOCK) {
+      const double kx = s.kvec[2 * k], ky = s.kvec[2 * k + 1];
+      double rc[2] = {0.0, 0.0}, rs[2] = {0.0, 0.0};
+      for (int t = 0; t < s.ntypes; t++) {
+        if (!s.irhok[t]) continue;
+        double c = 0.0, sn = 0.0;
+        const int i1 = s.ipf[t] + s.np[t];
+        for (int i = s.ipf[t]; i < i1; i++) {
+          double sv, cv;
+          sincos(kx * xs[2 * i] + ky * xs[2 * i + 1], &sv, &cv);
+          c += cv;
+          sn += sv;
+        }
+        rc[t] = c;
+        rs[t] = sn;
+        const int slot = s.jrhok + (s.irhok[t] - 1) * 2 * s.nk + 2 * k;
+        pw[slot] = c;
+        pw[slot + 1] = sn;
+      }
+      for (int it = 0; it < s.ntypes; it++)
+        for (int jt = 0; jt < s.ntypes; jt++) {
+          const int tp = tpair(it, jt);
+          if (k < s.nke[tp]) {
+            const double aux = s.vk[tp][k] * (rc[it] * rc[jt] + rs[it] * rs[jt]);
+            if (it == 0) pk0 += aux;
+            else pk1 += aux;
+            pkt += aux;
+          }
+        }
+    }
+  }
+
+  acc_ltf = block_sum(acc_ltf, red);
+  acc_h0 = block_sum(acc_h0, red);
+  acc_h1 = block_sum(acc_h1, red);
+  acc_v0 = block_sum(acc_v0, red);
+  acc_v1 = block_sum(acc_v1, red);
+  pk0 = block_sum(pk0, red);
+  pk1 = block_sum(pk1, red);
+  pkt = block_sum(pkt, red);
+  if (threadIdx.x == 0) {
+    double c0 = 0.0, c1 = 0.0, ct = 0.0;
+    for (int it = 0; it < s.ntypes; it++)
+      for (int jt = 0; jt < s.ntypes; jt++) {
+        const int tp = tpair(it, jt);
+        if (s.nke[tp] != 0) {
+          const double a = s.vk0[tp] * s.npnorm;
+          if (it == 0) c0 += a;
+          else c1 += a;
+          ct += a;
+        }
+      }
+    double* sc = wsw + L.off_sc;
+    sc[SC_LTF] = acc_ltf;
+    sc[SC_H0] = acc_h0;
+    sc[SC_H1] = acc_h1;
+    sc[SC_P0] = s.tailc[0] + acc_v0 + pk0 + c0;
+    sc[SC_P1] = s.tailc[1] + acc_v1 + pk1 + c1;
+    sc[SC_PT] = s.tailc_tot + acc_v0 + acc_v1 + pkt + ct;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_det
+// LDS: A[m*m] complex (column-major, row = orbital, column = particle; the
+// inverse overwrites it with row = particle, column = orbital), then a union
+// region reused by the phases.
+__global__ __launch_bounds__(QMC_BLOCK) void k_det(DevSys s, WsLayout L,
+                                                   const double* __restrict__ x,
+                                                   double* __restrict__ ws,
+                                                   double* __restrict__ Fbuf,
+                                                   double* __restrict__ Abuf, int kc,
+                                                   int tpp_log2) {
+  extern __shared__ double sm[];
+  __shared__ int s_piv[QMC_MAX_ORB];
+  __shared__ int s_l, s_sing;
+  __shared__ double s_pmax;
+  __shared__ zd s_pv, s_akk;
+  __shared__ double red[QMC_BLOCK / 64];
+  const int w = blockIdx.x, sp = blockIdx.y;
+  const int m = s.np[sp];
+  if (m == 0) return;
+  const int N = s.nptot, p0 = s.ipf[sp];
+  const int tid = threadIdx.x;
+  zd* A = (zd*)sm;
+  double* U = sm + 2 * m * m;
+  double* wsw = ws + (int64_t)w * L.per_walker;
+  double* dsc = wsw + L.off_det[sp];
+  zd* Ag = (zd*)(Abuf + (int64_t)w * L.a_per_walker + L.a_off[sp]);  // row-major [k][j]
+  zd* Fg = (zd*)(Fbuf + (int64_t)w * L.f_per_walker + L.f_off[sp]);   // [beta][l][j]
+  const double* qw = wsw + L.off_q;
+  if (tid == 0) s_sing = 0;
+
+  // 1. orbital matrix A(k,i) = exp(i g_k . q_i) (paho_orbitals, reptation.f:10298-10305)
+  for (int e = tid; e < m * m; e += QMC_BLOCK) {
+    const int k = e / m, i = e - k * m;
+    const double kr = qw[2 * (p0 + i)] * s.gvec[2 * k] + qw[2 * (p0 + i) + 1] * s.gvec[2 * k + 1];
+    double sv, cv;
+    sincos(kr, &sv, &cv);
+    const zd z = zmk(cv, sv);
+    A[i * m + k] = z;
+    Ag[k * m + i] = z;
+  }
+  __syncthreads();
+
+  // 2. LU with partial pivoting, LINPACK zgefa semantics: pivot = first row
+  // of maximal |re|+|im|; rows are swapped only right of the pivot column.
+  for (int k = 0; k < m - 1; k++) {
+    if (tid < 64) {
+      const int r = k + tid;
+      double v = -1.0;
+      int idx = 0x7fffffff;
+      if (r < m) { v = cabs1(A[k * m + r]); idx = r; }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(v, o, 64);
+        const int oi = __shfl_xor(idx, o, 64);
+        if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+      }
+      if (tid == 0) {
+        s_l = idx;
+        s_pmax = v;
+        s_pv = A[k * m + idx];
+        s_akk = A[k * m + k];
+        s_piv[k] = idx;
+        if (v == 0.0) s_sing = 1;
+      }
+    }
+    __syncthreads();
+    if (s_pmax == 0.0) {  // column already triangular (zgefa info = k)
+      __syncthreads();
+      continue;
+    }
+    const int l = s_l;
+    const zd pv = s_pv, akk = s_akk;
+    const zd tneg = zrcp(pv);
+    const zd t = zmk(-tneg.x, -tneg.y);
+    const int nr = m - k - 1;
+    for (int e = tid; e < nr + nr + 1; e += QMC_BLOCK) {
+      if (e < nr) {
+        const int j = k + 1 + e;
+        if (l != k) {
+          const zd a = A[j * m + k];
+          A[j * m + k] = A[j * m + l];
+          A[j * m + l] = a;
+        }
+      } else {
+        const int i = k + (e - nr);
+        if (i == k) A[k * m + k] = pv;
+        else {
+          const zd old = (i == l) ? akk : A[k * m + i];
+          A[k * m + i] = zmul(t, old);
+        }
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < nr * nr; e += QMC_BLOCK) {
+      const int jj = e / nr, ii = e - jj * nr;
+      const int i = k + 1 + ii, j = k + 1 + jj;
+      A[j * m + i] = zfma(A[j * m + k], A[k * m + i], A[j * m + i]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    s_piv[m - 1] = m - 1;
+    if (cabs1(A[(m - 1) * m + (m - 1)]) == 0.0) s_sing = 1;
+  }
+  __syncthreads();
+  // log|det| = sum log|U_kk| (zgedi det + log, reptation.f:10063, 9412-9430)
+  {
+    double v = 0.0;
+    if (tid < m) v = log(hypot(A[tid * m + tid].x, A[tid * m + tid].y));
+    v = block_sum(v, red);
+    if (tid == 0) {
+      dsc[DS_LOGDET] = v;
+      dsc[DS_SING] = s_sing ? 1.0 : 0.0;
+    }
+  }
+  if (s_sing) return;
+
+  // 3. inverse (zgedi job=01 ordering, reptation.f:9437-9468)
+  // 3a. inverse of U in place
+  zd* rowk = (zd*)U;
+  zd* colk = rowk + m;
+  for (int k = 0; k < m; k++) {
+    for (int e = tid; e < m; e += QMC_BLOCK) {
+      if (e > k) rowk[e] = A[e * m + k];
+      else if (e < k) colk[e] = A[k * m + e];
+      else colk[e] = A[k * m + k];
+    }
+    __syncthreads();
+    const zd tk = zrcp(colk[k]);
+    const int ncol = m - k - 1;
+    for (int e = tid; e < (k + 1) * (ncol + 1); e += QMC_BLOCK) {
+      const int jj = e / (k + 1), i = e - jj * (k + 1);
+      const zd ci = (i < k) ? zmul(zmk(-tk.x, -tk.y), colk[i]) : tk;
+      if (jj == 0) {
+        A[k * m + i] = ci;
+      } else {
+        const int j = k + jj;
+        const zd base = (i < k) ? A[j * m + i] : zmk(0.0, 0.0);
+        A[j * m + i] = zfma(rowk[j], ci, base);
+      }
+    }
+    __syncthreads();
+  }
+  // 3b. inverse(U) * inverse(L), right to left, with the column swaps
+  zd* work = (zd*)U;
+  zd* part = work + m;  // [4][m]
+  for (int k = m - 2; k >= 0; k--) {
+    for (int i = tid; i < m; i += QMC_BLOCK)
+      if (i > k) work[i] = A[k * m + i];
+    __syncthreads();
+    {
+      const int g = tid >> 6, i = tid & 63;
+      zd acc = zmk(0.0, 0.0);
+      if (i < m)
+        for (int j = k + 1 + g; j < m; j += 4) acc = zfma(work[j], A[j * m + i], acc);
+      if (i < m) part[g * m + i] = acc;
+    }
+    __syncthreads();
+    if (tid < m) {
+      const int i = tid;
+      zd v = (i <= k) ? A[k * m + i] : zmk(0.0, 0.0);
+      for (int g = 0; g < 4; g++) v = zadd(v, part[g * m + i]);
+      A[k * m + i] = v;
+    }
+    __syncthreads();
+    const int l = s_piv[k];
+    if (l != k) {
+      for (int i = tid; i < m; i += QMC_BLOCK) {
+        const zd a = A[k * m + i];
+        A[k * m + i] = A[l * m + i];
+        A[l * m + i] = a;
+      }
+    }
+    __syncthreads();
+  }
+
+  // 4. F_b = A^-1 diag(i g^b) A (zv, reptation.f:10182-10194), 4x4 tiles;
+  // A rows staged from the global copy kc at a time.
+  {
+    const int tl = tid & 15, tj = tid >> 4;
+    const int l0 = 4 * tl, j0 = 4 * tj;
+    zd Qx[4][4], Qy[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) { Qx[r][c] = zmk(0, 0); Qy[r][c] = zmk(0, 0); }
+    zd* Ac = (zd*)U;
+    for (int k0 = 0; k0 < m; k0 += kc) {
+      const int kn = min(kc, m - k0);
+      for (int e = tid; e < kn * m; e += QMC_BLOCK) Ac[e] = Ag[k0 * m + e];
+      __syncthreads();
+      if (l0 < m && j0 < m) {
+        for (int kk = 0; kk < kn; kk++) {
+          const int k = k0 + kk;
+          const double gx = s.gvec[2 * k], gy = s.gvec[2 * k + 1];
+          zd ai[4], aj[4];
+#pragma unroll
+          for (int r = 0; r < 4; r++) ai[r] = (l0 + r < m) ? A[k * m + l0 + r] : zmk(0, 0);
+#pragma unroll
+          for (int c = 0; c < 4; c++) aj[c] = (j0 + c < m) ? Ac[kk * m + j0 + c] : zmk(0, 0);
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+              const zd P = zmul(ai[r], aj[c]);
+              Qx[r][c].x = fma(gx, P.x, Qx[r][c].x);
+              Qx[r][c].y = fma(gx, P.y, Qx[r][c].y);
+              Qy[r][c].x = fma(gy, P.x, Qy[r][c].x);
+              Qy[r][c].y = fma(gy, P.y, Qy[r][c].y);
+            }
+        }
+      }
+      __syncthreads();
+    }
+    // F = i Q; store and keep the diagonal
+    double* dl = U;  // d[j][b] complex: 4 doubles per j
+    if (l0 < m && j0 < m) {
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const int l = l0 + r, j = j0 + c;
+          if (l < m && j < m) {
+            const zd fx = zmk(-Qx[r][c].y, Qx[r][c].x);
+            const zd fy = zmk(-Qy[r][c].y, Qy[r][c].x);
+            Fg[(0 * m + l) * m + j] = fx;
+            Fg[(1 * m + l) * m + j] = fy;
+            if (l == j) {
+              dl[4 * j + 0] = fx.x;
+              dl[4 * j + 1] = fx.y;
+              dl[4 * j + 2] = fy.x;
+              dl[4 * j + 3] = fy.y;
+            }
+          }
+        }
+    }
+  }
+  // diag(A^-1 d2A) real parts: dd[j] = (xx, xy, yy)
+  double* dd = U + 4 * m;
+  double* xs = dd + 3 * m;
+  for (int j = tid; j < m; j += QMC_BLOCK) {
+    double sxx = 0, sxy = 0, syy = 0;
+    for (int k = 0; k < m; k++) {
+      const zd P = zmul(A[k * m + j], Ag[k * m + j]);
+      const double gx = s.gvec[2 * k], gy = s.gvec[2 * k + 1];
+      sxx -= gx * gx * P.x;
+      sxy -= gx * gy * P.x;
+      syy -= gy * gy * P.x;
+    }
+    dd[3 * j] = sxx;
+    dd[3 * j + 1] = sxy;
+    dd[3 * j + 2] = syy;
+  }
+  const double* xw = x + (int64_t)w * 2 * N;
+  for (int k = tid; k < 2 * N; k += QMC_BLOCK) xs[k] = xw[k];
+  __syncthreads();
+
+  // 5. drift and the diagonal Laplacian terms for every particle
+  // (reptation.f:10195-10240), with the backflow Jacobian qa and the
+  // quasi-coordinate Laplacian qb regenerated from the pair table.
+  const double* dl = U;
+  const int tpp = 1 << tpp_log2;
+  const int lane = tid & (tpp - 1), grp = tid >> tpp_log2, ngrp = QMC_BLOCK >> tpp_log2;
+  double acc_g2 = 0.0, acc_h0 = 0.0, acc_h1 = 0.0;
+  double* gdet = dsc + DS_N;
+  double* Qd = gdet + 2 * N;
+  for (int i0 = 0; i0 < N; i0 += ngrp) {
+    const int ip = i0 + grp;
+    const bool act = ip < N;
+    const int ti = act ? ptype(s, ip) : 0;
+    const bool on = act && s.iub_flag[sp][ti];
+    const bool insp = ip >= p0 && ip < p0 + m;
+    double zxr = 0, zxi = 0, zyr = 0, zyi = 0, hq = 0, hd = 0;
+    double q11 = 0, q12 = 0, q22 = 0, bx = 0, by = 0;
+    if (on) {
+      const double xi = xs[2 * ip], yi = xs[2 * ip + 1];
+      for (int c = lane; c < N; c += tpp) {
+        if (c == ip) continue;
+        const int tb = s.tub[tpair(ti, ptype(s, c))];
+        if (tb < 0) continue;
+        PairGeo p = pair_geo(s, xi, yi, xs[2 * c], xs[2 * c + 1]);
+        double eta, deta, ddeta;
+        spline3(s, tb, p.ind, p.rem, eta, deta, ddeta);
+        const double cb = deta * p.byr;
+        const double mb11 = eta + cb * p.ux * p.ux, mb12 = cb * p.ux * p.uy,
+                     mb22 = eta + cb * p.uy * p.uy;
+        const double cc = ddeta + 3.0 * cb;
+        if (insp) {
+          q11 += mb11; q12 += mb12; q22 += mb22;
+          bx += cc * p.ux; by += cc * p.uy;
+        }
+        if (c >= p0 && c < p0 + m) {
+          const int jf = c - p0;
+          const double dxr = dl[4 * jf], dxi = dl[4 * jf + 1], dyr = dl[4 * jf + 2],
+                       dyi = dl[4 * jf + 3];
+          // qa(.,.,c,ip) = -mb
+          zxr -= dxr * mb11 + dyr * mb12;
+          zxi -= dxi * mb11 + dyi * mb12;
+          zyr -= dxr * mb12 + dyr * mb22;
+          zyi -= dxi * mb12 + dyi * mb22;
+          // qb(.,ip,c) = -cc u
+          hq -= cc * (dxr * p.ux + dyr * p.uy);
+          const double a11 = mb11 * mb11 + mb12 * mb12, a12 = mb12 * (mb11 + mb22),
+                       a22 = mb12 * mb12 + mb22 * mb22;
+          hd += dd[3 * jf] * a11 + 2.0 * dd[3 * jf + 1] * a12 + dd[3 * jf + 2] * a22;
+        }
+      }
+    }
+    for (int o = 1; o < tpp; o <<= 1) {
+      zxr += __shfl_xor(zxr, o, 64);
+      zxi += __shfl_xor(zxi, o, 64);
+      zyr += __shfl_xor(zyr, o, 64);
+      zyi += __shfl_xor(zyi, o, 64);
+      hq += __shfl_xor(hq, o, 64);
+      hd += __shfl_xor(hd, o, 64);
+      q11 += __shfl_xor(q11, o, 64);
+      q12 += __shfl_xor(q12, o, 64);
+      q22 += __shfl_xor(q22, o, 64);
+      bx += __shfl_xor(bx, o, 64);
+      by += __shfl_xor(by, o, 64);
+    }
+    if (act && lane == 0) {
+      if (on) {
+        if (insp) {
+          const int jf = ip - p0;
+          const double a11 = 1.0 + q11, a12 = q12, a22 = 1.0 + q22;
+          const double dxr = dl[4 * jf], dxi = dl[4 * jf + 1], dyr = dl[4 * jf + 2],
+                       dyi = dl[4 * jf + 3];
+          zxr += dxr * a11 + dyr * a12;
+          zxi += dxi * a11 + dyi * a12;
+          zyr += dxr * a12 + dyr * a22;
+          zyi += dxi * a12 + dyi * a22;
+          hq += dxr * bx + dyr * by;
+          const double b11 = a11 * a11 + a12 * a12, b12 = a12 * (a11 + a22),
+                       b22 = a12 * a12 + a22 * a22;
+          hd += dd[3 * jf] * b11 + 2.0 * dd[3 * jf + 1] * b12 + dd[3 * jf + 2] * b22;
+          Qd[3 * jf] = q11;
+          Qd[3 * jf + 1] = q12;
+          Qd[3 * jf + 2] = q22;
+        }
+        gdet[2 * ip] = -zxr;
+        gdet[2 * ip + 1] = -zyr;
+        acc_g2 += s.hbs2m[ti] * (zxi * zxi + zyi * zyi);
+        if (ti == 0) acc_h0 -= hq + hd;
+        else acc_h1 -= hq + hd;
+      } else {
+        gdet[2 * ip] = 0.0;
+        gdet[2 * ip + 1] = 0.0;
+      }
+    }
+  }
+  acc_g2 = block_sum(acc_g2, red);
+  acc_h0 = block_sum(acc_h0, red);
+  acc_h1 = block_sum(acc_h1, red);
+  if (tid == 0) {
+    dsc[DS_H0] = acc_h0;
+    dsc[DS_H1] = acc_h1;
+    dsc[DS_G2] = acc_g2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_cross: h_t += sum_{ip in t, idim} sum_{j,k} Re[(qa F)(qa F)] written as
+// sum_{rows r,s} M[r][s] S_t[r][s], rows r = (j, a) of the spin's particles,
+// M[(j,a),(k,b)] = Re(F_a(k,j) F_b(j,k)) (symmetric),
+// S_t = V_t V_t^T with V[(j,a),(ip,idim)] = qa(a,idim,j,ip).
+// Upper-triangle 8x8 tiles, one per thread; V staged 8 particles at a time.
+#define XC_CP 8
+__global__ __launch_bounds__(QMC_BLOCK) void k_cross(DevSys s, WsLayout L,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ ws,
+                                                     const double* __restrict__ Fbuf) {
+  extern __shared__ double sm[];
+  __shared__ double red[QMC_BLOCK / 64];
+  const int w = blockIdx.x, sp = blockIdx.y;
+  const int m = s.np[sp];
+  if (m == 0) return;
+  const int N = s.nptot, p0 = s.ipf[sp];
+  const int tid = threadIdx.x;
+  double* wsw = ws + (int64_t)w * L.per_walker;
+  double* dsc = wsw + L.off_det[sp];
+  if (dsc[DS_SING] != 0.0) return;
+  const double* Qd = dsc + DS_N + 2 * N;
+  const zd* Fg = (const zd*)(Fbuf + (int64_t)w * L.f_per_walker + L.f_off[sp]);
+  const int R = 2 * m;
+  const int nT = (R + 7) >> 3;
+  const int RP = nT * 8;
+  double* xs = sm;                 // [2N]
+  double* Vc = sm + 2 * N;         // [RP][2*XC_CP]
+  const double* xw = x + (int64_t)w * 2 * N;
+  for (int k = tid; k < 2 * N; k += QMC_BLOCK) xs[k] = xw[k];
+  // tile of this thread
+  const int ntile = nT * (nT + 1) / 2;
+  int TJ = 0, TK = 0;
+  {
+    int t = tid, row = 0;
+    while (row < nT && t >= nT - row) { t -= nT - row; row++; }
+    TJ = row;
+    TK = row + t;
+  }
+  const bool tact = tid < ntile;
+  double Mt[8][8];
+#pragma unroll
+  for (int a = 0; a < 8; a++)
+#pragma unroll
+    for (int b = 0; b < 8; b++) Mt[a][b] = 0.0;
+  if (tact) {
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      const int r = TJ * 8 + a;
+      const int j = r >> 1, ra = r & 1;
+      if (j >= m) continue;
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const int c = TK * 8 + b;
+        const int k = c >> 1, cb = c & 1;
+        if (k >= m) continue;
+        const zd f1 = Fg[(ra * m + k) * m + j];
+        const zd f2 = Fg[(cb * m + j) * m + k];
+        Mt[a][b] = f1.x * f2.x - f1.y * f2.y;
+      }
+    }
+  }
+  __syncthreads();
+  double xc[2] = {0.0, 0.0};
+  for (int t = 0; t < s.ntypes; t++) {
+    if (!s.iub_flag[sp][t] || s.np[t] == 0) continue;
+    double St[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+      for (int b = 0; b < 8; b++) St[a][b] = 0.0;
+    const int ia = s.ipf[t], ib = s.ipf[t] + s.np[t];
+    for (int c0 = ia; c0 < ib; c0 += XC_CP) {
+      const int cn = min(XC_CP, ib - c0);
+      // V chunk: rows (j,a), columns (ip,idim)
+      for (int e = tid; e < RP * XC_CP / 2; e += QMC_BLOCK) {
+        const int j = e / XC_CP, pc = e - j * XC_CP;
+        double v11 = 0, v12 = 0, v21 = 0, v22 = 0;  // [a][idim]
+        const int ip = c0 + pc;
+        if (j < m && pc < cn) {
+          const int jp = p0 + j;
+          if (jp == ip) {
+            v11 = 1.0 + Qd[3 * j];
+            v12 = Qd[3 * j + 1];
+            v21 = v12;
+            v22 = 1.0 + Qd[3 * j + 2];
+          } else {
+            const int tb = s.tub[tpair(sp, ptype(s, ip))];
+            if (tb >= 0) {
+              PairGeo p = pair_geo(s, xs[2 * jp], xs[2 * jp + 1], xs[2 * ip], xs[2 * ip + 1]);
+              double eta, deta, ddeta;
+              spline3(s, tb, p.ind, p.rem, eta, deta, ddeta);
+              const double cb = deta * p.byr;
+              v11 = -(eta + cb * p.ux * p.ux);
+              v12 = -(cb * p.ux * p.uy);
+              v21 = v12;
+              v22 = -(eta + cb * p.uy * p.uy);
+            }
+          }
+        }
+        Vc[(2 * j) * (2 * XC_CP) + 2 * pc] = v11;
+        Vc[(2 * j) * (2 * XC_CP) + 2 * pc + 1] = v12;
+        Vc[(2 * j + 1) * (2 * XC_CP) + 2 * pc] = v21;
+        Vc[(2 * j + 1) * (2 * XC_CP) + 2 * pc + 1] = v22;
+      }
+      __syncthreads();
+      if (tact) {
+        for (int col = 0; col < 2 * cn; col++) {
+          double va[8], vb[8];
+#pragma unroll
+          for (int a = 0; a < 8; a++) va[a] = Vc[(TJ * 8 + a) * (2 * XC_CP) + col];
+#pragma unroll
+          for (int b = 0; b < 8; b++) vb[b] = Vc[(TK * 8 + b) * (2 * XC_CP) + col];
+#pragma unroll
+          for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int b = 0; b < 8; b++) St[a][b] = fma(va[a], vb[b], St[a][b]);
+        }
+      }
+      __syncthreads();
+    }
+    double acc = 0.0;
+    if (tact) {
+#pragma unroll
+      for (int a = 0; a < 8; a++)
+#pragma unroll
+        for (int b = 0; b < 8; b++) acc = fma(Mt[a][b], St[a][b], acc);
+      if (TJ != TK) acc *= 2.0;
+    }
+    xc[t] = block_sum(acc, red);
+  }
+  if (tid == 0) {
+    dsc[DS_X0] = xc[0];
+    dsc[DS_X1] = xc[1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_final: one wavefront per walker.
+__global__ __launch_bounds__(QMC_BLOCK) void k_final(DevSys s, WsLayout L, double* __restrict__ ws,
+                                                     double* __restrict__ props,
+                                                     double* __restrict__ gout,
+                                                     double* __restrict__ hout,
+                                                     double* __restrict__ grad2,
+                                                     int32_t* __restrict__ status, int nw) {
+  const int w = blockIdx.x * (QMC_BLOCK / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= nw) return;
+  const int N = s.nptot;
+  double* wsw = ws + (int64_t)w * L.per_walker;
+  const double* sc = wsw + L.off_sc;
+  double ltf = sc[SC_LTF];
+  bool sing = false, guard = false;
+  if (s.has_det) {
+    for (int sp = 0; sp < s.ntypes; sp++) {
+      if (s.np[sp] == 0) continue;
+      const double* dsc = wsw + L.off_det[sp];
+      ltf -= dsc[DS_LOGDET];
+      if (dsc[DS_SING] != 0.0) sing = true;
+    }
+    // underflow guard of slater_excitations (reptation.f:10087-10092)
+    guard = !(exp(-ltf * 2.0) > 0.0);
+  }
+  const bool use_det = s.has_det && !guard && !sing;
+  double h[2] = {sc[SC_H0], sc[SC_H1]};
+  double g2ph = 0.0;
+  if (use_det) {
+    for (int sp = 0; sp < s.ntypes; sp++) {
+      if (s.np[sp] == 0) continue;
+      const double* dsc = wsw + L.off_det[sp];
+      h[0] += dsc[DS_H0] + dsc[DS_X0];
+      h[1] += dsc[DS_H1] + dsc[DS_X1];
+      g2ph += dsc[DS_G2];
+    }
+  }
+  double* gw = gout + (int64_t)w * 2 * N;
+  double g2t[2] = {0.0, 0.0}, ggs = 0.0;
+  for (int i = lane; i < N; i += 64) {
+    double gx = wsw[L.off_gj + 2 * i], gy = wsw[L.off_gj + 2 * i + 1];
+    if (use_det) {
+      for (int sp = 0; sp < s.ntypes; sp++) {
+        if (s.np[sp] == 0) continue;
+        const double* gd = wsw + L.off_det[sp] + DS_N;
+        gx += gd[2 * i];
+        gy += gd[2 * i + 1];
+      }
+    }
+    const int t = ptype(s, i);
+    const double x2 = gx * gx + gy * gy;
+    if (s.adrift != 0.0) {
+      // drift cutoff (kinetic, reptation.f:2747-2760)
+      const double vari = 1.0 / (2.0 * s.hbs2m[t] * s.delta);
+      double gg = x2 * s.adrift * s.delta * s.delta * vari;
+      gg = (-1.0 + sqrt(1.0 + 2.0 * gg)) / gg;
+      ggs += gg;
+      gx *= gg;
+      gy *= gg;
+    }
+    g2t[t] += x2;
+    gw[2 * i] = gx;
+    gw[2 * i + 1] = gy;
+  }
+  g2t[0] = wave_sum(g2t[0]);
+  g2t[1] = wave_sum(g2t[1]);
+  ggs = wave_sum(ggs);
+  if (lane == 0) {
+    double* p = props + (int64_t)w * s.n_props;
+    double kin0 = 0.0;
+    for (int t = 0; t < s.ntypes; t++) {
+      const double kin = s.hbs2m[t] * (h[t] - g2t[t]);
+      kin0 += kin;
+      p[s.jkin[t]] = kin / s.np[t];
+      p[s.jun[t]] = 1.0 / sqrt(g2t[t]);
+      hout[(int64_t)w * 2 + t] = h[t];
+    }
+    if (s.ntypes == 1) hout[(int64_t)w * 2 + 1] = 0.0;
+    p[s.jkin0] = kin0 / s.npnorm;
+    p[s.jgg] = (s.adrift == 0.0) ? 1.0 : ggs / s.npnorm;
+    if (use_det) grad2[w] = g2ph;
+    const double g2v = grad2[w];
+    p[s.jefp] = (s.ntheta != 0) ? g2v / s.npnorm : 0.0;
+    p[s.jltf] = ltf;
+    for (int t = 0; t < s.ntypes; t++) p[s.jpot[t]] = sc[SC_P0 + t] / s.np[t];
+    p[s.jpot0] = (sc[SC_PT] + s.v0) / s.npnorm;
+    p[s.jetot] = p[s.jpot0] + p[s.jkin0] + p[s.jefp];
+    p[s.je2] = p[s.jetot] * p[s.jetot];
+    if (status) status[w] = (guard ? 1 : 0) | (sing ? 2 : 0);
+  }
+}

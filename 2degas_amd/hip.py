@@ -1,0 +1,153 @@
+"""ctypes binding of lib2degas_hip.so (the HIP walker-evaluation library).
+
+The library is the compute path; this module only marshals arguments.  It
+raises when the shared library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib2degas_hip.so")
+
+# every entry point declared in include/qmc_hip.h
+EXPORTS = ("qmc_ctx_create", "qmc_ctx_destroy", "qmc_ctx_info", "qmc_set_twist",
+           "qmc_twist_gvec", "qmc_eval_batch", "qmc_eval_batch_device", "qmc_strerror",
+           "qmc_last_error", "qmc_flops_per_eval", "qmc_timing", "qmc_timing_read")
+
+_lib = None
+
+
+class QmcError(RuntimeError):
+    def __init__(self, code, where):
+        L = lib()
+        msg = L.qmc_last_error().decode() or L.qmc_strerror(code).decode()
+        super().__init__(f"{where}: {msg} (code {code})")
+        self.code = code
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        dp, ip, vp = abi.c_double_p, abi.c_int32_p, C.c_void_p
+        L.qmc_ctx_create.argtypes = [C.POINTER(abi.QmcTables), C.c_int, C.POINTER(vp)]
+        L.qmc_ctx_destroy.argtypes = [vp]
+        L.qmc_ctx_info.argtypes = [vp, C.POINTER(abi.QmcInfo)]
+        L.qmc_set_twist.argtypes = [vp, dp, C.c_int]
+        L.qmc_twist_gvec.argtypes = [dp, C.c_int, dp, C.c_int, C.c_int, C.c_int, dp]
+        L.qmc_eval_batch.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp, ip]
+        L.qmc_eval_batch_device.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, C.c_int, vp]
+        L.qmc_strerror.restype = C.c_char_p
+        L.qmc_strerror.argtypes = [C.c_int]
+        L.qmc_last_error.restype = C.c_char_p
+        L.qmc_flops_per_eval.restype = C.c_double
+        L.qmc_flops_per_eval.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.qmc_timing.argtypes = [vp, C.c_int]
+        L.qmc_timing_read.argtypes = [vp, dp, C.POINTER(C.c_int)]
+        _lib = L
+    return _lib
+
+
+def twist_gvec(kvec, el, ntheta, ith, jth):
+    """g-vectors of twist (ith, jth) as tabc builds them (host code)."""
+    kvec = np.ascontiguousarray(kvec, dtype=np.float64).reshape(-1, 2)
+    el = np.ascontiguousarray(el, dtype=np.float64)
+    out = np.zeros((2 * kvec.shape[0] + 1, 2))
+    rc = lib().qmc_twist_gvec(abi.dptr(kvec), kvec.shape[0], abi.dptr(el), ntheta, ith, jth,
+                              abi.dptr(out))
+    if rc:
+        raise QmcError(rc, "qmc_twist_gvec")
+    return out
+
+
+def flops_per_eval(nup, ndown, nk_ewald):
+    return lib().qmc_flops_per_eval(nup, ndown, nk_ewald)
+
+
+class Context:
+    """A device context for one deck (qmc_ctx_create + qmc_set_twist)."""
+
+    def __init__(self, deck, twist=(1, 1), gvec=None, device=0):
+        self.deck = deck
+        s, keep = deck.pack()
+        h = C.c_void_p()
+        rc = lib().qmc_ctx_create(C.byref(s), device, C.byref(h))
+        if rc:
+            raise QmcError(rc, "qmc_ctx_create")
+        self.h = h
+        if gvec is None:
+            gvec = twist_gvec(deck.kvec, deck.el, deck.ntheta, twist[0], twist[1])
+        self.set_twist(gvec)
+        inf = abi.QmcInfo()
+        lib().qmc_ctx_info(self.h, C.byref(inf))
+        self.info = inf.as_dict()
+
+    def set_twist(self, gvec):
+        self.gvec = np.ascontiguousarray(gvec, dtype=np.float64).reshape(-1, 2)
+        rc = lib().qmc_set_twist(self.h, abi.dptr(self.gvec), self.gvec.shape[0])
+        if rc:
+            raise QmcError(rc, "qmc_set_twist")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().qmc_ctx_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def eval(self, x, grad2_in=None):
+        """Host-pointer batched evaluation (qmc_eval_batch)."""
+        N = self.deck.nptot
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, N, 2)
+        nw = x.shape[0]
+        props = np.zeros((nw, self.info["n_props"]))
+        g = np.zeros((nw, N, 2))
+        h = np.zeros((nw, 2))
+        g2 = np.zeros(nw) if grad2_in is None else np.array(grad2_in, dtype=np.float64)
+        st = np.zeros(nw, np.int32)
+        rc = lib().qmc_eval_batch(self.h, nw, abi.dptr(x), abi.dptr(props), abi.dptr(g),
+                                  abi.dptr(h), abi.dptr(g2), abi.iptr(st))
+        if rc:
+            raise QmcError(rc, "qmc_eval_batch")
+        return dict(props=props, g=g, h=h, grad2_ph=g2, status=st)
+
+    def timing(self, enable):
+        """Reset and enable (or disable) per-kernel event timing."""
+        rc = lib().qmc_timing(self.h, 1 if enable else 0)
+        if rc:
+            raise QmcError(rc, "qmc_timing")
+
+    def timing_read(self):
+        """Mean ms per launch of (pairs, det, cross, final) and the launch count."""
+        ms = np.zeros(4)
+        n = C.c_int(0)
+        rc = lib().qmc_timing_read(self.h, abi.dptr(ms), C.byref(n))
+        if rc:
+            raise QmcError(rc, "qmc_timing_read")
+        return ms, n.value
+
+    def eval_device(self, x, props, g, h, grad2, status=None, precision=abi.QMC_PREC_FP64,
+                    stream=None):
+        """Device-pointer evaluation; arguments are torch tensors on the GPU
+        (or raw integer addresses).  Only enqueues on `stream`."""
+        def ptr(t):
+            if t is None:
+                return None
+            return C.c_void_p(t if isinstance(t, int) else t.data_ptr())
+        st = None if stream is None else C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
+        nw = x.shape[0]
+        rc = lib().qmc_eval_batch_device(self.h, nw, ptr(x), ptr(props), ptr(g), ptr(h), ptr(grad2),
+                                         ptr(status), precision, st)
+        if rc:
+            raise QmcError(rc, "qmc_eval_batch_device")

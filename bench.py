@@ -1,0 +1,184 @@
+"""Throughput of batched local-energy evaluation on one MI355X.
+
+One step = one evaluation (compute_properties(1) for every walker) of a
+batch of walkers whose positions are already resident in HBM.  Workload:
+N=122 (61+61) electrons, rs=10, Slater-Jastrow-backflow with u2 + u3,
+Ewald over the first 200 k-shells (900 k-vectors), 4096 walkers per launch,
+synthetic positions (a jittered 12x11 grid per walker) and a generated deck.
+
+Prints one JSON line (see DESIGN.md for the fields).
+"""
+import argparse
+import importlib
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+
+
+def lattice_configs(nw, nup, ndown, L, seed):
+    import numpy as np
+    n = nup + ndown
+    nx = int(math.ceil(math.sqrt(n)))
+    ny = int(math.ceil(n / nx))
+    gx = (np.arange(nx) + 0.5) / nx * L - L / 2
+    gy = (np.arange(ny) + 0.5) / ny * L - L / 2
+    sites = np.array([(a, b) for a in gx for b in gy])
+    rng = np.random.default_rng(seed)
+    x = np.empty((nw, n, 2))
+    for w in range(nw):
+        x[w] = sites[rng.permutation(len(sites))[:n]] + rng.normal(0, 0.15 * L / nx, size=(n, 2))
+    return x
+
+
+def kernel_flops(nup, ndown, nk_ewald):
+    """Algorithmic flops per walker of each kernel (pairs, det, cross, final)."""
+    n = nup + ndown
+    pairs = 160.0 * n * (n - 1) / 2 + 5.0 * n * nk_ewald + 20.0 * nk_ewald
+    det = sum(24.0 * m ** 3 + 37.0 * m * m for m in (nup, ndown))
+    cross = sum(24.0 * n * m * m for m in (nup, ndown))
+    return [pairs, det, cross, 0.0]
+
+
+def cpu_baseline(deckdir, x, seconds_target=12.0):
+    """The reference Fortran (oracle/_ref/refeval, timing mode) run as one
+    independent process per host core on a bounded sample."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    if not pyoracle.have_ref():
+        return None
+    cores = min(os.cpu_count() or 1, 16)
+    # calibrate: one process, a few evaluations
+    work = tempfile.mkdtemp(prefix="degas_cpu_")
+    raw, _ = pyoracle.run_refeval(deckdir, os.path.join(work, "cal"), 3, x[:64], nsteps=4)
+    t1 = float(np.frombuffer(raw[20:28], dtype=np.float64)[0]) / 4
+    nev = max(4, int(seconds_target / max(t1, 1e-6)))
+    procs = []
+    for p in range(cores):
+        wd = os.path.join(work, f"p{p}")
+        os.makedirs(wd, exist_ok=True)
+        for fn in os.listdir(deckdir):
+            src = os.path.join(deckdir, fn)
+            if os.path.isfile(src):
+                with open(src, "rb") as a, open(os.path.join(wd, fn), "wb") as b:
+                    b.write(a.read())
+        np.ascontiguousarray(x[(p * 64) % len(x):(p * 64) % len(x) + 64]).tofile(os.path.join(wd, "cfg.bin"))
+        with open(os.path.join(wd, "refeval.ctl"), "w") as f:
+            f.write(f"3\n1 1\n{nev}\n0.0\ncfg.bin\nout.bin\n")
+    t0 = time.time()
+    for p in range(cores):
+        wd = os.path.join(work, f"p{p}")
+        procs.append(subprocess.Popen([os.path.join(pyoracle.REF_DIR, "refeval")], cwd=wd,
+                                      stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL))
+    for pr in procs:
+        pr.wait()
+    wall = time.time() - t0
+    total = cores * nev
+    with open(os.path.join(pyoracle.REF_DIR, "COMPILER")) as f:
+        comp = f.read().strip()
+    return {"value": total / wall, "unit": "evals/s", "cores": cores, "kind": "reference",
+            "sample": f"{nev} compute_properties(1) per process x {cores} processes, "
+                      f"same deck and synthetic walkers, {wall:.1f} s wall ({comp})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nw", type=int, default=4096)
+    ap.add_argument("--nup", type=int, default=61)
+    ap.add_argument("--ndown", type=int, default=61)
+    ap.add_argument("--rs", type=float, default=10.0)
+    ap.add_argument("--shells", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every walker batch is independent, and this build "
+              "runs on a single MI355X (--gpus 1).", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    hip = importlib.import_module("2degas_amd.hip")
+    deck = importlib.import_module("2degas_amd.deck")
+    dg = importlib.import_module("2degas_amd.deckgen")
+    tmp = tempfile.mkdtemp(prefix="degas_bench_")
+    deckdir = os.path.join(tmp, "deck")
+    info = dg.make_deck(deckdir, "bench", args.nup, args.ndown, args.rs, args.shells)
+    d = deck.read_deck(deckdir)
+    N = d.nptot
+    x = lattice_configs(args.nw, args.nup, args.ndown, d.el[0], 12345)
+    ctx = hip.Context(d, twist=(1, 1), device=0)
+    dev = torch.device("cuda:0")
+    xt = torch.tensor(x, dtype=torch.float64, device=dev)
+    props = torch.zeros((args.nw, ctx.info["n_props"]), dtype=torch.float64, device=dev)
+    g = torch.zeros((args.nw, N, 2), dtype=torch.float64, device=dev)
+    h = torch.zeros((args.nw, 2), dtype=torch.float64, device=dev)
+    g2 = torch.zeros(args.nw, dtype=torch.float64, device=dev)
+    st = torch.zeros(args.nw, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        ctx.eval_device(xt, props, g, h, g2, st, stream=stream)
+    torch.cuda.synchronize()
+    ctx.timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.eval_device(xt, props, g, h, g2, st, stream=stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_k, nl = ctx.timing_read()
+    ctx.timing(False)
+    dt = t1 - t0
+    evals = args.nw * args.steps
+    value = evals / dt
+    guard = int((st.cpu().numpy() & 1).sum())
+    kf = kernel_flops(args.nup, args.ndown, info["nk_ewald"])
+    names = ["pairs", "det", "cross", "final"]
+    dom = int(np.argmax(ms_k))
+    achieved = kf[dom] * args.nw / (ms_k[dom] * 1e-3) / 1e12
+    total_flops = hip.flops_per_eval(args.nup, args.ndown, info["nk_ewald"])
+    out = {
+        "metric": "local-energy evals/s at N=122",
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"N={N} ({args.nup}+{args.ndown}) rs={args.rs} Slater-Jastrow-backflow "
+                               f"u2+u3, Ewald {args.shells} k-shells ({info['nk_ewald']} k), "
+                               f"{args.nw} walkers per launch, FP64 path",
+                   "walkers": args.nw, "N": N, "nk_ewald": info["nk_ewald"], "precision": "fp64"},
+        "roofline": {"bound": "mfma", "kernel": f"k_{names[dom]}", "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": None},
+        "kernel_ms": dict(zip(names, [float(v) for v in ms_k])),
+        "eval_tflops": total_flops * value / 1e12,
+        "eval_frac_fp64_peak": total_flops * value / 1e12 / FP64_PEAK_TFLOPS,
+        "guard_fired": guard,
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(deckdir, x, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -148,6 +148,17 @@ int qmc_eval_batch_device(qmc_ctx* ctx, int nw, const double* d_x,
                           double* d_grad2_ph, int32_t* d_status, int precision,
                           void* stream);
 
+/*
+ * Per-kernel timing with HIP events recorded on the launch stream around each
+ * kernel of every evaluation while enabled (no counterpart in the reference,
+ * which has no timers).  qmc_timing(ctx, 1) resets and enables,
+ * qmc_timing(ctx, 0) disables.  qmc_timing_read waits for the recorded work
+ * and returns the mean milliseconds per launch of the kernels in the order
+ * pairs, determinant, cross term, final (ms[4]) and the launches counted.
+ */
+int qmc_timing(qmc_ctx* ctx, int enable);
+int qmc_timing_read(qmc_ctx* ctx, double* ms, int* nlaunch);
+
 /* Human-readable text of an error code, and of the last HIP error seen. */
 const char* qmc_strerror(int code);
 const char* qmc_last_error(void);

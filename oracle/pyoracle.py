@@ -156,6 +156,14 @@ def run_refeval(deckdir, workdir, mode, x, twist=(1, 1), delta=0.0, nsteps=None,
         return f.read(), r.stdout.decode()
 
 
+def read_gvec_dump(workdir):
+    """g-vectors the harness used (twistg or input's list)."""
+    with open(os.path.join(workdir, "gvec.bin"), "rb") as f:
+        b = f.read()
+    n = int(np.frombuffer(b[:4], dtype=np.int32)[0])
+    return np.frombuffer(b[4:4 + 16 * n], dtype=np.float64).reshape(n, 2).copy()
+
+
 def parse_eval(raw, nptot):
     head = np.frombuffer(raw[:20], dtype=np.int32)
     ns, npt, nt, nk, nr = [int(v) for v in head]

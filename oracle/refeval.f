@@ -40,6 +40,12 @@ c move (reptation.f:4617) + metropolis_test (reptation.f:2622).
       read(11,'(a)') outfile
       close(11)
       if(ntheta.ne.0)call twistg(ith0,jth0)
+c the g-vector list in use (twistg or input's untwisted list)
+      open(14,file='gvec.bin',access='stream',form='unformatted'
+     &    ,status='replace')
+      write(14) 2*nk+1
+      write(14) ((gvec(idim,k),idim=1,2),k=1,2*nk+1)
+      close(14)
       nk_out=0
       if(nrhok.ne.0)nk_out=nk
       open(12,file=cfgfile,access='stream',form='unformatted'

@@ -1,0 +1,78 @@
+import importlib
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+DECKS = os.path.join(GOLDEN, "decks")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+def pkg(name=""):
+    return importlib.import_module("2degas_amd" + ("." + name if name else ""))
+
+
+def load_deck(name):
+    return pkg("deck").read_deck(os.path.join(DECKS, name))
+
+
+def load_golden(fn):
+    return dict(np.load(os.path.join(GOLDEN, fn)))
+
+
+def rel_err(a, b, floor=0.0):
+    """max |a-b| / max(max |b|, floor) (norm-wise relative error); NaN and
+    infinite reference entries (per-type slots of an empty type) must match
+    in kind."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    m = np.isfinite(b)
+    if not np.array_equal(np.isnan(a), np.isnan(b)) or not np.array_equal(np.isinf(a), np.isinf(b)):
+        return float("inf")
+    if not np.any(m):
+        return 0.0
+    den = max(np.max(np.abs(b[m])), floor, 1e-300)
+    return float(np.max(np.abs(a[m] - b[m])) / den)
+
+
+def compare_eval(out, gold, n_scalar, rtol, label=""):
+    """Compare an evaluation (oracle or HIP) with a reference dump:
+    each scalar slot per walker to rtol relative (norm-wise over walkers),
+    drift per walker norm-wise, laplacian, grad2_ph and rhok."""
+    sc = out["props"][:, :n_scalar]
+    gs = gold["scalars"]
+    # the fixed-phase term is rounding noise where the orbitals are real
+    # (periodic twists); measure it against the kinetic-energy scale
+    ekin_scale = float(np.max(np.abs(gs[:, 4])))
+    errs = {}
+    for k in range(n_scalar):
+        if k == 2:  # acceptance slot is not written by the evaluation
+            continue
+        floor = ekin_scale if k == n_scalar - 1 else 0.0
+        errs[f"slot{k + 1}"] = rel_err(sc[:, k], gs[:, k], floor)
+    errs["g"] = max(rel_err(out["g"][w], gold["g"][w]) for w in range(gs.shape[0]))
+    nt = gold["h"].shape[1]
+    errs["h"] = rel_err(out["h"][:, :nt], gold["h"])
+    if np.any(gold["grad2_ph"] != 0):
+        npn = out["g"].shape[1]
+        errs["grad2_ph"] = rel_err(out["grad2_ph"], gold["grad2_ph"], ekin_scale * npn)
+    if gold["rhok"].size:
+        errs["rhok"] = rel_err(out["props"][:, n_scalar:], gold["rhok"])
+    bad = {k: v for k, v in errs.items() if not v <= rtol}
+    assert not bad, f"{label}: errors above {rtol}: {bad}"
+    return errs
+
+
+@pytest.fixture(scope="session")
+def oracle_mod():
+    import pyoracle
+    pyoracle.build()
+    return pyoracle

@@ -1,0 +1,141 @@
+"""Write the golden fixtures of the parity tests.
+
+Runs the reference Fortran (oracle/_ref/refeval, built from the reference
+sources by oracle/build_ref.sh) in this container and stores its inputs and
+outputs as small .npz files beside this script.  The decks the fixtures use
+live under tests/golden/decks/:
+
+  n26      the reference's shipped N=26 deck (rs=1, backflow, u2, u3, 4x4 twists)
+  n26g     the same deck at the Gamma point (ntwists line dropped: no efp)
+  n26p     plain Slater-Jastrow: the backflow table replaced by zeros, Gamma
+  free26   free plane waves: zero backflow, no u2/u3, Gamma
+  g58      generated N=58 (29+29), rs=5, u2+u3+backflow, 60 k-shells
+  g122     generated N=122 (61+61), rs=10, u2+u3+backflow, 200 k-shells
+  g61p     generated polarized N=61 (+ an empty second type), rs=10
+
+Usage: python tests/golden/make_golden.py   (needs oracle/_ref; minutes)
+"""
+import importlib
+import os
+import shutil
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+deck = importlib.import_module("2degas_amd.deck")
+deckgen = importlib.import_module("2degas_amd.deckgen")
+import pyoracle as po  # noqa: E402
+
+DECKS = os.path.join(HERE, "decks")
+WORK = "/tmp/degas_golden"
+
+
+def derive_n26_variants():
+    src = os.path.join(DECKS, "n26")
+    for name in ("n26g", "n26p", "free26"):
+        dst = os.path.join(DECKS, name)
+        shutil.rmtree(dst, ignore_errors=True)
+        os.makedirs(dst)
+        for fn in ("n26.u2", "n26.u3", "n26.b", "n26.v", "n26.k", "runid", "seed",
+                   "n26.up.x.res.0", "n26.down.x.res.0"):
+            shutil.copy(os.path.join(src, fn), dst)
+        with open(os.path.join(src, "n26.sy")) as fh:
+            lines = fh.read().splitlines()
+        out = []
+        for ln in lines:
+            w = ln.split()
+            if not w:
+                continue
+            if w[0] == "ntwists":
+                continue
+            if name == "free26" and w[0] in ("u2", "u3"):
+                continue
+            if name in ("n26p", "free26") and w[0] == "backflow":
+                ln = ln.replace("n26.b", "zero.b")
+            out.append(ln)
+        with open(os.path.join(dst, "n26.sy"), "w") as fh:
+            fh.write("\n".join(out) + "\n")
+        if name in ("n26p", "free26"):
+            t = deck.read_table(os.path.join(src, "n26.b"))
+            rows = np.zeros_like(t.coef)
+            deckgen.write_table(os.path.join(dst, "zero.b"), rows, t.drt, "zero")
+
+
+def make_generated():
+    deckgen.make_deck(os.path.join(DECKS, "g58"), "g58", 29, 29, 5.0, 60)
+    deckgen.make_deck(os.path.join(DECKS, "g122"), "g122", 61, 61, 10.0, 200)
+    deckgen.make_deck(os.path.join(DECKS, "g61p"), "g61p", 61, 0, 10.0, 200)
+
+
+def eval_fixture(name, nconf, seed, twist=(1, 1), extra=None):
+    d = deck.read_deck(os.path.join(DECKS, name))
+    x = deckgen.random_configs(nconf, d.nptot, d.el[0], seed)
+    if extra is not None:
+        x = np.concatenate([extra, x])
+    raw, _ = po.run_refeval(os.path.join(DECKS, name), os.path.join(WORK, name), 1, x, twist=twist)
+    r = po.parse_eval(raw, d.nptot)
+    out = os.path.join(HERE, f"eval_{name}_t{twist[0]}{twist[1]}.npz")
+    np.savez_compressed(out, x=x, scalars=r["scalars"], g=r["g"], h=r["h"],
+                        grad2_ph=r["grad2_ph"], rhok=r["rhok"], twist=np.array(twist))
+    print("wrote", out, x.shape)
+
+
+def replay_fixture(name, nsteps, every, delta, x0):
+    d = deck.read_deck(os.path.join(DECKS, name))
+    raw, _ = po.run_refeval(os.path.join(DECKS, name), os.path.join(WORK, name + "_r"), 2,
+                            x0[None], twist=(1, 1), delta=delta, nsteps=nsteps)
+    r = po.parse_replay(raw, d.nptot, nsteps)
+    ck = np.arange(0, nsteps, every)
+    out = os.path.join(HERE, f"replay_{name}.npz")
+    np.savez_compressed(out, delta=delta, every=every, ck_seeds=r["seeds"][ck],
+                        ck_x=r["x_old"][ck], p=r["p"], age=r["age"], ltf_new=r["ltf_new"],
+                        e_new=r["e_new"], first_seed=r["seeds"][0])
+    print("wrote", out, "accept rate", np.mean(r["age"] == 0))
+
+
+def main():
+    if not po.have_ref():
+        sys.exit("oracle/_ref missing: run oracle/build_ref.sh first")
+    os.makedirs(WORK, exist_ok=True)
+    derive_n26_variants()
+    make_generated()
+    d26 = deck.read_deck(os.path.join(DECKS, "n26"))
+    shipped = deck.read_configs([os.path.join(DECKS, "n26", f) for f in
+                                 ("n26.up.x.res.0", "n26.down.x.res.0")], [13, 13])
+    shipped1 = deck.read_configs([os.path.join(DECKS, "n26", f) for f in
+                                  ("n26.up.x.res.1", "n26.down.x.res.1")], [13, 13])
+    shipped = np.concatenate([shipped, shipped1])
+    eval_fixture("n26", 30, 11, (1, 1), extra=shipped)
+    eval_fixture("n26", 8, 12, (2, 3))
+    eval_fixture("n26", 8, 13, (4, 4))
+    eval_fixture("n26g", 16, 14)
+    eval_fixture("n26p", 16, 15)
+    eval_fixture("free26", 8, 16)
+    eval_fixture("g58", 16, 17)
+    eval_fixture("g122", 6, 18)
+    eval_fixture("g61p", 6, 19)
+    # twist lists written by twistg for a few twists (pins the heap sort)
+    gv = {}
+    for ij in ((1, 1), (2, 3), (4, 4), (3, 1)):
+        x = deckgen.random_configs(1, d26.nptot, d26.el[0], 1)
+        raw, _ = po.run_refeval(os.path.join(DECKS, "n26"), os.path.join(WORK, "tw"), 1, x, twist=ij)
+        gv[f"t{ij[0]}{ij[1]}"] = po.read_gvec_dump(os.path.join(WORK, "tw"))
+    raw, _ = po.run_refeval(os.path.join(DECKS, "n26g"), os.path.join(WORK, "tw"), 1, x)
+    gv["gamma"] = po.read_gvec_dump(os.path.join(WORK, "tw"))
+    np.savez_compressed(os.path.join(HERE, "twist_n26.npz"), **gv)
+    replay_fixture("n26", 10000, 100, 0.05, shipped[0])
+    # N=58 starts from a jittered 8x8 grid (58 of its 64 sites)
+    L58 = deck.read_deck(os.path.join(DECKS, "g58")).el[0]
+    g = (np.arange(8) + 0.5) / 8 * L58 - L58 / 2
+    sites = np.array([(a, b) for a in g for b in g])
+    rng = np.random.default_rng(21)
+    x58 = sites[rng.permutation(64)[:58]] + rng.normal(0, 0.05 * L58 / 8, size=(58, 2))
+    replay_fixture("g58", 1000, 100, 2.0, x58)
+
+
+if __name__ == "__main__":
+    main()

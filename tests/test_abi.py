@@ -1,0 +1,75 @@
+"""The C-ABI library loads and exports every entry point of
+include/qmc_hip.h; host-only entry points checked without a GPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, load_deck, load_golden, pkg
+
+
+def header_symbols():
+    with open(os.path.join(ROOT, "include", "qmc_hip.h")) as f:
+        txt = f.read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(qmc_\w+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_entry_point():
+    hip = pkg("hip")
+    L = hip.lib()
+    syms = header_symbols()
+    assert len(syms) >= 12
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(hip.EXPORTS)
+
+
+def test_twist_gvec_matches_reference_lists():
+    """qmc_twist_gvec (tabc association) reproduces twistg's lists for the
+    shipped n26 deck, heap-sort ties included, and input's Gamma list."""
+    hip = pkg("hip")
+    d = load_deck("n26")
+    gv = load_golden("twist_n26.npz")
+    for key, ref in gv.items():
+        if key == "gamma":
+            got = hip.twist_gvec(d.kvec, d.el, 0, 0, 0)
+        else:
+            got = hip.twist_gvec(d.kvec, d.el, d.ntheta, int(key[1]), int(key[2]))
+        assert np.array_equal(got, ref), key
+
+
+def test_flops_formula():
+    """Algorithmic flops per evaluation: the documented counts."""
+    hip = pkg("hip")
+    assert abs(hip.flops_per_eval(13, 13, 56) / 1e6 - 0.389) < 0.002
+    assert abs(hip.flops_per_eval(29, 29, 232) / 1e6 - 3.91) < 0.02
+    assert abs(hip.flops_per_eval(61, 61, 900) / 1e6 - 34.7) < 0.1
+    assert abs(hip.flops_per_eval(61, 0, 900) / 1e6 - 11.6) < 0.1
+
+
+def test_error_strings():
+    L = pkg("hip").lib()
+    for code in range(7):
+        assert L.qmc_strerror(code)
+
+
+def test_deck_reader_rejects_out_of_path_keywords(tmp_path):
+    deck = pkg("deck")
+    (tmp_path / "runid").write_text("x\n")
+    (tmp_path / "x.sy").write_text("ndim 2\ngofr up up 1\nend\n")
+    with pytest.raises(ValueError):
+        deck.read_deck(str(tmp_path))
+
+
+def test_deck_pack_layout():
+    d = load_deck("n26")
+    s, keep = d.pack()
+    assert s.ntypes == 2 and s.np[0] == 13 and s.np[1] == 13
+    assert s.ntab == 4 and s.nk == 56 and s.ntheta == 4
+    ut = keep[0]
+    # row ngrid+1.. of the v table hold vk, then vk0 (reptation.f:4716-4724)
+    t = d.tables[3]
+    assert np.array_equal(ut[3, 0, t.ngrid + 1:t.ngrid + 1 + len(t.vk)], t.vk)
