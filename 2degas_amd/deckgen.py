@@ -206,6 +206,22 @@ def random_configs(nconf, nptot, L, seed):
     return rng.uniform(-L / 2, L / 2, size=(nconf, nptot, 2))
 
 
+def lattice_configs(nconf, nptot, L, seed, jitter=0.15):
+    """Positions on a jittered square grid: each configuration takes nptot of
+    the ceil(sqrt(nptot))^2 cells in random order and displaces them by a
+    Gaussian of width jitter * cell."""
+    n = int(math.ceil(math.sqrt(nptot)))
+    m = int(math.ceil(nptot / n))
+    gx = (np.arange(n) + 0.5) / n * L - L / 2
+    gy = (np.arange(m) + 0.5) / m * L - L / 2
+    sites = np.array([(a, b) for a in gx for b in gy])
+    rng = np.random.default_rng(seed)
+    x = np.empty((nconf, nptot, 2))
+    for c in range(nconf):
+        x[c] = sites[rng.permutation(len(sites))[:nptot]] + rng.normal(0, jitter * L / n, size=(nptot, 2))
+    return x
+
+
 def write_configs(dirpath, runid, x, counts, names=("up", "down"), suffix=".0"):
     """Configuration files in read_conf's layout (reptation.f:2474-2485)."""
     off = 0

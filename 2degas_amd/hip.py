@@ -30,9 +30,18 @@ class QmcError(RuntimeError):
 
 
 def lib():
-    """Load the HIP library (raises if it has not been built)."""
+    """Load the HIP library (raises if it has not been built).
+
+    torch's wheel bundles a HIP runtime with the same soname as the system
+    one; loading torch first makes the library bind to that runtime so the
+    process holds one HIP runtime and torch-allocated device buffers can be
+    passed to the device-pointer entry points."""
     global _lib
     if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} missing: build it with __graft_entry__.build()")
         L = C.CDLL(LIB_PATH)

@@ -4,7 +4,7 @@ One step = one evaluation (compute_properties(1) for every walker) of a
 batch of walkers whose positions are already resident in HBM.  Workload:
 N=122 (61+61) electrons, rs=10, Slater-Jastrow-backflow with u2 + u3,
 Ewald over the first 200 k-shells (900 k-vectors), 4096 walkers per launch,
-synthetic positions (a jittered 12x11 grid per walker) and a generated deck.
+synthetic positions (a jittered 12x12 grid per walker) and a generated deck.
 
 Prints one JSON line (see DESIGN.md for the fields).
 """
@@ -22,21 +22,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
-
-
-def lattice_configs(nw, nup, ndown, L, seed):
-    import numpy as np
-    n = nup + ndown
-    nx = int(math.ceil(math.sqrt(n)))
-    ny = int(math.ceil(n / nx))
-    gx = (np.arange(nx) + 0.5) / nx * L - L / 2
-    gy = (np.arange(ny) + 0.5) / ny * L - L / 2
-    sites = np.array([(a, b) for a in gx for b in gy])
-    rng = np.random.default_rng(seed)
-    x = np.empty((nw, n, 2))
-    for w in range(nw):
-        x[w] = sites[rng.permutation(len(sites))[:n]] + rng.normal(0, 0.15 * L / nx, size=(n, 2))
-    return x
 
 
 def kernel_flops(nup, ndown, nk_ewald):
@@ -117,7 +102,7 @@ def main():
     info = dg.make_deck(deckdir, "bench", args.nup, args.ndown, args.rs, args.shells)
     d = deck.read_deck(deckdir)
     N = d.nptot
-    x = lattice_configs(args.nw, args.nup, args.ndown, d.el[0], 12345)
+    x = dg.lattice_configs(args.nw, N, d.el[0], 12345)
     ctx = hip.Context(d, twist=(1, 1), device=0)
     dev = torch.device("cuda:0")
     xt = torch.tensor(x, dtype=torch.float64, device=dev)

@@ -46,7 +46,11 @@ def rel_err(a, b, floor=0.0):
 def compare_eval(out, gold, n_scalar, rtol, label=""):
     """Compare an evaluation (oracle or HIP) with a reference dump:
     each scalar slot per walker to rtol relative (norm-wise over walkers),
-    drift per walker norm-wise, laplacian, grad2_ph and rhok."""
+    drift per walker norm-wise, laplacian, grad2_ph and rhok.
+    Slot 6 (e2 = E_L^2) carries twice E_L's relative error and is held to
+    2 rtol; h, the per-type laplacian sum, adds large terms of both signs
+    when electrons come close and is held to 5 rtol (the kinetic energy
+    built from it is held to rtol)."""
     sc = out["props"][:, :n_scalar]
     gs = gold["scalars"]
     # the fixed-phase term is rounding noise where the orbitals are real
@@ -57,10 +61,10 @@ def compare_eval(out, gold, n_scalar, rtol, label=""):
         if k == 2:  # acceptance slot is not written by the evaluation
             continue
         floor = ekin_scale if k == n_scalar - 1 else 0.0
-        errs[f"slot{k + 1}"] = rel_err(sc[:, k], gs[:, k], floor)
+        errs[f"slot{k + 1}"] = rel_err(sc[:, k], gs[:, k], floor) / (2.0 if k == 5 else 1.0)
     errs["g"] = max(rel_err(out["g"][w], gold["g"][w]) for w in range(gs.shape[0]))
     nt = gold["h"].shape[1]
-    errs["h"] = rel_err(out["h"][:, :nt], gold["h"])
+    errs["h"] = rel_err(out["h"][:, :nt], gold["h"]) / 5.0
     if np.any(gold["grad2_ph"] != 0):
         npn = out["g"].shape[1]
         errs["grad2_ph"] = rel_err(out["grad2_ph"], gold["grad2_ph"], ekin_scale * npn)
