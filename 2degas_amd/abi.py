@@ -93,3 +93,28 @@ def iptr(a):
     """Pointer to a C-contiguous int32 numpy array."""
     assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(c_int32_p)
+
+
+class QmcWalkers(C.Structure):
+    _fields_ = [
+        ("nw", C.c_int32),
+        ("x", C.c_void_p),
+        ("g", C.c_void_p),
+        ("h", C.c_void_p),
+        ("props", C.c_void_p),
+        ("grad2_ph", C.c_void_p),
+        ("age", C.c_void_p),
+        ("rng", C.c_void_p),
+        ("blk", C.c_void_p),
+    ]
+
+
+def limbs_to_state(limbs):
+    """48-bit LCG state from the four 12-bit limbs savern writes."""
+    l1, l2, l3, l4 = (int(v) for v in limbs)
+    return ((l1 * 4096 + l2) * 4096 + l3) * 4096 + l4
+
+
+def state_to_limbs(x):
+    x = int(x)
+    return [(x >> 36) & 4095, (x >> 24) & 4095, (x >> 12) & 4095, x & 4095]

@@ -1,0 +1,68 @@
+// qmc_internal.h -- host-side context shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "qmc_dev.h"
+#include "qmc_hip.h"
+
+extern thread_local std::string g_last_error;
+
+inline int hip_fail(hipError_t e, const char* what) {
+  char buf[256];
+  snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+  g_last_error = buf;
+  return QMC_ERR_HIP;
+}
+#define HIPCHK(call)                                       \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);      \
+  } while (0)
+
+inline int arg_fail(const char* msg, int code = QMC_ERR_ARG) {
+  g_last_error = msg;
+  return code;
+}
+
+struct qmc_ctx {
+  int device = 0;
+  DevSys s{};
+  qmc_info info{};
+  double4* d_tab = nullptr;
+  double* d_kvec = nullptr;
+  double* d_vk = nullptr;
+  double* d_gvec = nullptr;
+  // workspace sized for ws_nw walkers
+  int ws_nw = 0;
+  double* d_ws = nullptr;
+  double* d_F = nullptr;
+  double* d_A = nullptr;
+  WsLayout L{};
+  // staging for the host-pointer entry point
+  int st_nw = 0;
+  double *d_x = nullptr, *d_props = nullptr, *d_g = nullptr, *d_h = nullptr, *d_g2 = nullptr;
+  int32_t* d_status = nullptr;
+  hipStream_t stream = nullptr;
+  int kc = 16;
+  size_t lds_pairs = 0, lds_det = 0, lds_cross = 0;
+  int tpp_pairs = 0, tpp_det = 0;
+  // per-kernel event timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;  // 5 events per recorded launch
+  int ev_used = 0;
+  // VMC scratch: proposal buffers for vmc_nw walkers
+  int vmc_nw = 0;
+  double *d_xn = nullptr, *d_gn = nullptr, *d_hn = nullptr, *d_pn = nullptr, *d_wold = nullptr;
+  int32_t *d_stn = nullptr, *d_sticky = nullptr;
+};
+
+// Queue one batched evaluation (k_pairs, k_det, k_cross, k_final) on st.
+int launch_eval(qmc_ctx* c, int nw, const double* d_x, double* d_props, double* d_g, double* d_h,
+                double* d_g2, int32_t* d_status, hipStream_t st);
+void free_vmc(qmc_ctx* c);
+
+

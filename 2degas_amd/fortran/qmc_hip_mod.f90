@@ -1,0 +1,162 @@
+! qmc_hip_mod.f90 -- ISO_C_BINDING interface to lib2degas_hip.so.
+!
+! Mirrors include/qmc_hip.h one to one: bind(C) derived types for the
+! qmc_tables / qmc_info / qmc_walkers structs and bind(C) interfaces for
+! every entry point.  A Fortran host (the reference driver, or any other)
+! uses this module to hand its deck tables to the device once and then to
+! evaluate walkers through the C-ABI.
+module qmc_hip_mod
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: QMC_OK = 0, QMC_ERR_ARG = 1, QMC_ERR_LIMIT = 2, &
+       QMC_ERR_SINGULAR = 3, QMC_ERR_HIP = 4, QMC_ERR_NOTWIST = 5, QMC_ERR_STATE = 6
+  integer(c_int), parameter :: QMC_PREC_FP64 = 0, QMC_PREC_MIXED = 1
+
+  type, bind(C) :: qmc_tables
+     integer(c_int32_t) :: ndim, ntypes
+     integer(c_int32_t) :: np(2)
+     real(c_double) :: hbs2m(2), el(2), drt
+     integer(c_int32_t) :: ntab, ldut
+     type(c_ptr) :: ut, ngrid, tail, iexp, nk_ewald
+     integer(c_int32_t) :: iu2(4), iu3(4), iub(4), iv2(4)
+     real(c_double) :: v2value(4), v0
+     integer(c_int32_t) :: nk
+     type(c_ptr) :: kvec
+     integer(c_int32_t) :: irhok(2), nrhok, ntheta
+     real(c_double) :: adrift, delta
+  end type qmc_tables
+
+  type, bind(C) :: qmc_info
+     integer(c_int32_t) :: nptot, npnorm, n_scalar_props, n_props
+     integer(c_int32_t) :: jetot, jltf, jacc, jpot0, jkin0, je2, jgg, jefp, jrhok
+     integer(c_int32_t) :: jpot(2), jkin(2), jun(2)
+     integer(c_int32_t) :: ng, has_det
+  end type qmc_info
+
+  type, bind(C) :: qmc_walkers
+     integer(c_int32_t) :: nw
+     type(c_ptr) :: x, g, h, props, grad2_ph, age, rng, blk
+  end type qmc_walkers
+
+  interface
+     integer(c_int) function qmc_ctx_create(tables, device, ctx) bind(C, name='qmc_ctx_create')
+       import :: c_int, c_ptr, qmc_tables
+       type(qmc_tables), intent(in) :: tables
+       integer(c_int), value :: device
+       type(c_ptr), intent(out) :: ctx
+     end function qmc_ctx_create
+
+     integer(c_int) function qmc_ctx_destroy(ctx) bind(C, name='qmc_ctx_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function qmc_ctx_destroy
+
+     integer(c_int) function qmc_ctx_info(ctx, info) bind(C, name='qmc_ctx_info')
+       import :: c_int, c_ptr, qmc_info
+       type(c_ptr), value :: ctx
+       type(qmc_info), intent(out) :: info
+     end function qmc_ctx_info
+
+     integer(c_int) function qmc_set_twist(ctx, gvec, ng) bind(C, name='qmc_set_twist')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: gvec(*)
+       integer(c_int), value :: ng
+     end function qmc_set_twist
+
+     integer(c_int) function qmc_twist_gvec(kvec, nk, el, ntheta, ith, jth, gvec) &
+          bind(C, name='qmc_twist_gvec')
+       import :: c_int, c_double
+       real(c_double), intent(in) :: kvec(*), el(*)
+       integer(c_int), value :: nk, ntheta, ith, jth
+       real(c_double), intent(out) :: gvec(*)
+     end function qmc_twist_gvec
+
+     integer(c_int) function qmc_eval_batch(ctx, nw, x, props, g, h, grad2_ph, status) &
+          bind(C, name='qmc_eval_batch')
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nw
+       real(c_double), intent(in) :: x(*)
+       real(c_double), intent(inout) :: props(*), g(*), h(*), grad2_ph(*)
+       integer(c_int32_t), intent(out) :: status(*)
+     end function qmc_eval_batch
+
+     integer(c_int) function qmc_eval_batch_device(ctx, nw, x, props, g, h, grad2_ph, status, &
+          precision, stream) bind(C, name='qmc_eval_batch_device')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, x, props, g, h, grad2_ph, status, stream
+       integer(c_int), value :: nw, precision
+     end function qmc_eval_batch_device
+
+     integer(c_int) function qmc_vmc_start(ctx, w, stream) bind(C, name='qmc_vmc_start')
+       import :: c_int, c_ptr, qmc_walkers
+       type(c_ptr), value :: ctx, stream
+       type(qmc_walkers), intent(in) :: w
+     end function qmc_vmc_start
+
+     integer(c_int) function qmc_vmc_sweep(ctx, w, nsteps, delta, mage, bm_order, trace_p, &
+          trace_age, stream) bind(C, name='qmc_vmc_sweep')
+       import :: c_int, c_ptr, c_double, qmc_walkers
+       type(c_ptr), value :: ctx, trace_p, trace_age, stream
+       type(qmc_walkers), intent(in) :: w
+       integer(c_int), value :: nsteps, bm_order
+       real(c_double), value :: delta, mage
+     end function qmc_vmc_sweep
+
+     integer(c_int) function qmc_vmc_status(ctx, nw, sticky) bind(C, name='qmc_vmc_status')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nw
+       integer(c_int32_t), intent(out) :: sticky(*)
+     end function qmc_vmc_status
+
+     integer(c_int) function qmc_timing(ctx, enable) bind(C, name='qmc_timing')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: enable
+     end function qmc_timing
+
+     integer(c_int) function qmc_timing_read(ctx, ms, nlaunch) bind(C, name='qmc_timing_read')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: ms(4)
+       integer(c_int), intent(out) :: nlaunch
+     end function qmc_timing_read
+
+     type(c_ptr) function qmc_strerror(code) bind(C, name='qmc_strerror')
+       import :: c_int, c_ptr
+       integer(c_int), value :: code
+     end function qmc_strerror
+
+     type(c_ptr) function qmc_last_error() bind(C, name='qmc_last_error')
+       import :: c_ptr
+     end function qmc_last_error
+
+     real(c_double) function qmc_flops_per_eval(nup, ndown, nk_ewald) &
+          bind(C, name='qmc_flops_per_eval')
+       import :: c_int, c_double
+       integer(c_int), value :: nup, ndown, nk_ewald
+     end function qmc_flops_per_eval
+  end interface
+
+contains
+
+  ! Text of the library's last error as a Fortran string.
+  function qmc_error_text() result(txt)
+    character(len=256) :: txt
+    character(kind=c_char), pointer :: cs(:)
+    type(c_ptr) :: p
+    integer :: i
+    txt = ' '
+    p = qmc_last_error()
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, cs, [256])
+    do i = 1, 256
+       if (cs(i) == c_null_char) exit
+       txt(i:i) = cs(i)
+    end do
+  end function qmc_error_text
+
+end module qmc_hip_mod

@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "lib2degas_hip.so")
 # every entry point declared in include/qmc_hip.h
 EXPORTS = ("qmc_ctx_create", "qmc_ctx_destroy", "qmc_ctx_info", "qmc_set_twist",
            "qmc_twist_gvec", "qmc_eval_batch", "qmc_eval_batch_device", "qmc_strerror",
-           "qmc_last_error", "qmc_flops_per_eval", "qmc_timing", "qmc_timing_read")
+           "qmc_last_error", "qmc_flops_per_eval", "qmc_timing", "qmc_timing_read",
+           "qmc_vmc_start", "qmc_vmc_sweep", "qmc_vmc_status")
 
 _lib = None
 
@@ -59,6 +60,10 @@ def lib():
         L.qmc_flops_per_eval.restype = C.c_double
         L.qmc_flops_per_eval.argtypes = [C.c_int, C.c_int, C.c_int]
         L.qmc_timing.argtypes = [vp, C.c_int]
+        L.qmc_vmc_start.argtypes = [vp, C.POINTER(abi.QmcWalkers), vp]
+        L.qmc_vmc_sweep.argtypes = [vp, C.POINTER(abi.QmcWalkers), C.c_int, C.c_double,
+                                    C.c_double, C.c_int, vp, vp, vp]
+        L.qmc_vmc_status.argtypes = [vp, C.c_int, ip]
         L.qmc_timing_read.argtypes = [vp, dp, C.POINTER(C.c_int)]
         _lib = L
     return _lib
@@ -160,3 +165,62 @@ class Context:
                                          ptr(status), precision, st)
         if rc:
             raise QmcError(rc, "qmc_eval_batch_device")
+
+
+class Walkers:
+    """Device walker state of a batched VMC run (torch tensors on cuda)."""
+
+    def __init__(self, ctx, x, rng_states, block_sums=True):
+        import torch
+        self.ctx = ctx
+        dev = torch.device("cuda", 0)
+        N = ctx.deck.nptot
+        nw = x.shape[0]
+        f64 = torch.float64
+        self.nw = nw
+        self.x = torch.tensor(np.asarray(x, dtype=np.float64).reshape(nw, N, 2), device=dev)
+        self.g = torch.zeros((nw, N, 2), dtype=f64, device=dev)
+        self.h = torch.zeros((nw, 2), dtype=f64, device=dev)
+        self.props = torch.zeros((nw, ctx.info["n_props"]), dtype=f64, device=dev)
+        self.grad2 = torch.zeros(nw, dtype=f64, device=dev)
+        self.age = torch.zeros(nw, dtype=f64, device=dev)
+        self.rng = torch.tensor(np.asarray(rng_states, dtype=np.int64), device=dev)
+        self.blk = torch.zeros((nw, ctx.info["n_props"]), dtype=f64, device=dev) if block_sums else None
+        s = abi.QmcWalkers()
+        s.nw = nw
+        for name in ("x", "g", "h", "props", "age", "rng", "blk"):
+            t = getattr(self, name)
+            setattr(s, name, t.data_ptr() if t is not None else None)
+        s.grad2_ph = self.grad2.data_ptr()
+        self.s = s
+
+    def _stream(self, stream):
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream()
+        return C.c_void_p(st.cuda_stream)
+
+    def start(self, stream=None):
+        rc = lib().qmc_vmc_start(self.ctx.h, C.byref(self.s), self._stream(stream))
+        if rc:
+            raise QmcError(rc, "qmc_vmc_start")
+
+    def sweep(self, nsteps, delta, mage=1e30, bm_order=0, trace=False, stream=None):
+        import torch
+        tp = ta = None
+        if trace:
+            tp = torch.zeros((nsteps, self.nw), dtype=torch.float64, device=self.x.device)
+            ta = torch.zeros((nsteps, self.nw), dtype=torch.float64, device=self.x.device)
+        rc = lib().qmc_vmc_sweep(self.ctx.h, C.byref(self.s), nsteps, delta, mage, bm_order,
+                                 C.c_void_p(tp.data_ptr()) if trace else None,
+                                 C.c_void_p(ta.data_ptr()) if trace else None,
+                                 self._stream(stream))
+        if rc:
+            raise QmcError(rc, "qmc_vmc_sweep")
+        return tp, ta
+
+    def status(self):
+        st = np.zeros(self.nw, np.int32)
+        rc = lib().qmc_vmc_status(self.ctx.h, self.nw, abi.iptr(st))
+        if rc:
+            raise QmcError(rc, "qmc_vmc_status")
+        return st

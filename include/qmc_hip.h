@@ -149,6 +149,58 @@ int qmc_eval_batch_device(qmc_ctx* ctx, int nw, const double* d_x,
                           void* stream);
 
 /*
+ * Walker state of a batched VMC run, all in device memory and owned by the
+ * caller (nw walkers; N = nptot).  Walker w plays the role of MPI rank w of
+ * the reference: its own old configuration (/c_old/, ewald.h:49-52), age,
+ * fixed-phase scratch and its own stream of the 48-bit LCG rannyu
+ * (reptation.f:5247-5291), stored as the 48-bit integer
+ * l1*2^36 + l2*2^24 + l3*2^12 + l4 of the four limbs.
+ */
+typedef struct qmc_walkers {
+  int32_t nw;
+  double* x;        /* [nw][N][2]  x_old                                   */
+  double* g;        /* [nw][N][2]  g_old                                   */
+  double* h;        /* [nw][2]     h_old                                   */
+  double* props;    /* [nw][n_props] p_old                                 */
+  double* grad2_ph; /* [nw]        fixed-phase term of the last evaluation */
+  double* age;      /* [nw]        consecutive rejections                  */
+  int64_t* rng;     /* [nw]        LCG state                               */
+  double* blk;      /* [nw][n_props] block sums of p_old (averages(2)), or NULL */
+} qmc_walkers;
+
+/*
+ * Start a VMC run from the configurations already in w->x: evaluate them and
+ * accept unconditionally, as read_conf does (reptation.f:2486-2495:
+ * compute_properties(1), then metropolis_test(1)).  Device pointers, queued
+ * on `stream`.
+ */
+int qmc_vmc_start(qmc_ctx* ctx, const qmc_walkers* w, void* stream);
+
+/*
+ * nsteps VMC steps for every walker: move (reptation.f:4617-4680: Box-Muller
+ * drift-diffusion proposal of all electrons, evaluation, reverse-move weight)
+ * then metropolis_test (reptation.f:2622-2659: a uniform is drawn only when
+ * p < 1; accept if p >= u; copy new into old), then averages(2)
+ * (reptation.f:4300-4305: blk += p_old) when w->blk is not NULL.
+ * delta is the time step (sigma, reptation.f:855-870: var = 2 hbs2m delta);
+ * mage forces acceptance after that many consecutive rejections.
+ * bm_order selects which rannyu() of the Box-Muller expression is drawn
+ * first (0: the cos() argument, as the amdflang build of the reference does).
+ * trace_p / trace_age (optional, [nsteps][nw]) receive p and age per step.
+ */
+int qmc_vmc_sweep(qmc_ctx* ctx, const qmc_walkers* w, int nsteps, double delta,
+                  double mage, int bm_order, double* trace_p, double* trace_age,
+                  void* stream);
+
+/*
+ * Wait for the queued VMC work and copy each walker's sticky status (bit 1:
+ * underflow guard fired at some step, bit 2: singular orbital matrix) to
+ * host memory; returns QMC_ERR_SINGULAR if any walker hit a singular matrix
+ * (where the reference stops, reptation.f:10318).
+ */
+int qmc_vmc_status(qmc_ctx* ctx, int nw, int32_t* sticky);
+
+/*
  * Per-kernel timing with HIP events recorded on the launch stream around each
  * kernel of every evaluation while enabled (no counterpart in the reference,
  * which has no timers).  qmc_timing(ctx, 1) resets and enables,

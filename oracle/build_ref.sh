@@ -49,5 +49,21 @@ $FC $FFLAGS -c refmain.f -o refmain.o
 mkdir -p "$OUT"
 $FC $FFLAGS refeval.o reptation.o mpif.o -o "$OUT/refeval"
 $FC $FFLAGS refmain.o reptation.o mpif.o -o "$OUT/refqmc"
+# drop-in build: the reference driver with its compute_properties replaced
+# by the GPU forwarder (2degas_amd/fortran); needs lib2degas_hip.so
+PKG="$(dirname "$HERE")/2degas_amd"
+if [ -f "$PKG/lib2degas_hip.so" ]; then
+  sed -e 's/^      subroutine compute_properties(ifdist)/      subroutine compute_properties_cpu(ifdist)/' \
+      reptation.f > reptation_dropin.f
+  grep -q 'subroutine compute_properties_cpu' reptation_dropin.f || { echo "build_ref: drop-in rename failed" >&2; exit 1; }
+  $FC $FFLAGS -c reptation_dropin.f -o reptation_dropin.o
+  $FC $FFLAGS -c "$PKG/fortran/qmc_hip_mod.f90" -o qmc_hip_mod.o
+  cp "$PKG/fortran/compute_properties_hip.f" .
+  $FC $FFLAGS -c compute_properties_hip.f -o compute_properties_hip.o
+  $FC $FFLAGS refmain.o reptation_dropin.o mpif.o compute_properties_hip.o qmc_hip_mod.o \
+      -L"$PKG" -l2degas_hip -Wl,-rpath,'$ORIGIN/../../2degas_amd' -Wl,-rpath,/opt/rocm/lib \
+      -o "$OUT/refqmc_hip"
+  echo "build_ref: built $OUT/refqmc_hip (reference driver + GPU compute_properties)"
+fi
 echo "$FC $($FC --version 2>/dev/null | head -1) $OPT" > "$OUT/COMPILER"
 echo "build_ref: built $OUT/refeval $OUT/refqmc with $FC $OPT"

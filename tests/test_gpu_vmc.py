@@ -1,0 +1,68 @@
+"""Device VMC step (move + metropolis_test + averages) against the
+reference's recorded move sequence."""
+import numpy as np
+import pytest
+
+from conftest import load_deck, load_golden, pkg
+
+pytestmark = pytest.mark.gpu
+
+
+def _replay(name):
+    hip = pkg("hip")
+    abi = pkg("abi")
+    gold = load_golden(f"replay_{name}.npz")
+    d = load_deck(name)
+    ctx = hip.Context(d, twist=(1, 1))
+    every = int(gold["every"])
+    states = [abi.limbs_to_state(l) for l in gold["ck_seeds"]]
+    wk = hip.Walkers(ctx, gold["ck_x"], states)
+    wk.start()
+    tp, ta = wk.sweep(every, float(gold["delta"]), trace=True)
+    import torch
+    torch.cuda.synchronize()
+    assert np.all(wk.status() == 0)
+    p = tp.cpu().numpy().T.reshape(-1)       # walker-major = checkpoint order
+    age = ta.cpu().numpy().T.reshape(-1)
+    return gold, p, age
+
+
+@pytest.mark.parametrize("name", ["n26", "g58"])
+def test_vmc_accept_sequence_matches_reference(name):
+    """Teacher-forced replay: every 100-move segment of the reference run
+    starts from the reference's recorded walker and RNG state; the device
+    reproduces every accept/reject decision and every acceptance
+    probability."""
+    print(f"\n[gpu] vmc replay {name}", flush=True)
+    gold, p, age = _replay(name)
+    n = len(gold["p"])
+    acc_ref = gold["age"][:n] == 0
+    acc = age[:n] == 0
+    mism = int(np.sum(acc != acc_ref))
+    relp = float(np.max(np.abs(p[:n] - gold["p"]) / np.maximum(gold["p"], 1e-300)))
+    print(f"   moves {n} mismatches {mism} max rel p {relp:.2e} acc rate {acc.mean():.3f}", flush=True)
+    assert mism == 0
+    assert relp < 1e-8
+
+
+def test_vmc_block_sums_accumulate_old_properties():
+    """averages(2): blk accumulates p_old after every step; with 3 steps the
+    energy slot of blk equals the sum of the traced old energies."""
+    print("\n[gpu] vmc block sums", flush=True)
+    import torch
+    hip = pkg("hip")
+    abi = pkg("abi")
+    d = load_deck("n26")
+    ctx = hip.Context(d, twist=(1, 1))
+    x = pkg("deckgen").lattice_configs(8, d.nptot, d.el[0], 4)
+    sts = [abi.limbs_to_state([3535, 3467, 3262, (3099 + 2 * w) % 4096]) for w in range(8)]
+    wk = hip.Walkers(ctx, x, sts)
+    wk.start()
+    e = []
+    for _ in range(3):
+        wk.sweep(1, 0.05)
+        e.append(wk.props[:, 0].clone())
+    torch.cuda.synchronize()
+    tot = sum(v.cpu().numpy() for v in e)
+    assert np.allclose(wk.blk[:, 0].cpu().numpy(), tot, rtol=1e-14, atol=0)
+    assert np.all(wk.blk[:, 2].cpu().numpy() <= 3.0)
