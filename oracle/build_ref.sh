@@ -39,6 +39,12 @@ sed -e 's/mgrid= 901 ,/mgrid=2100 ,/' \
 for pat in 'mgrid=2100' 'morbit=  64,mnk=2048' 'mstack=1000' 'm_props_in_stack=4000'; do
   grep -q "$pat" "$TMP/ewald.h" || { echo "build_ref: limit patch '$pat' did not apply" >&2; exit 1; }
 done
+# Fortran does not short-circuit .and.: with der_nskip = 0 (no 'der' deck
+# line) amdflang evaluates mod(istp,0) in vmc and rmc (reptation.f:1001,
+# 1343) and traps.  Guard the divisor; the branch stays false as intended.
+sed -i -e 's/mod(istp,der_nskip)/mod(istp,max(der_nskip,1))/' \
+       -e 's/mod(conta,der_nskip)/mod(conta,max(der_nskip,1))/' "$TMP/reptation.f"
+[ "$(grep -c 'max(der_nskip,1)' "$TMP/reptation.f")" = 2 ] || { echo "build_ref: der_nskip guard did not apply" >&2; exit 1; }
 cp "$HERE/refeval.f" "$HERE/refmain.f" "$TMP/"
 cd "$TMP"
 FFLAGS="$OPT -w"
