@@ -26,12 +26,21 @@ struct DevSys {
   // table (0-based, -1 = none) per type pair for u2, u3, backflow, v2
   int tu2[3], tu3[3], tub[3], tv2[3];
   int has_u2, has_u3, has_b, has_v2;
+  // per table kind (TK_U2, TK_U3, TK_B, TK_V) and type pair: first row of
+  // the table in `tab` and its ngrid (-1 row = no table); iexp of v tables
+  int trow[4][3], tng[4][3], vexp[3];
   int iub_flag[2][2];      // determinant of spin s couples particle type t
   double v2value[3];
   double tailc[2], tailc_tot;  // pair-potential tail constants
   // k space
   int nk, nrhok, irhok[2];
   const double* kvec;      // [nk][2]
+  // factorized phases: k = (kxv[kxi], kyv[kyi]) over few distinct components;
+  // nkxv = 0 selects the direct sincos path
+  int nkxv, nkyv, kchunk;
+  const double* kxv;
+  const double* kyv;
+  const int* kxy;          // [nk] kxi | kyi << 16
   int nke[3];              // nk_ewald of the v2 table of each type pair
   const double* vk[3];     // [nke]
   double vk0[3];
@@ -77,12 +86,66 @@ __device__ __forceinline__ zd zrcp(zd a) {
   return zd{a.x / d, -a.y / d};
 }
 
+// sin and cos of x for |x| < 2^30 (absolute error below 1.4e-16).  Cody-Waite
+// reduction r = x - n pi/2 with pi/2 split over three doubles (the first
+// product is exact inside the FMA), then the Taylor series of sin r and cos r
+// on |r| <= pi/4 through r^17 and r^18 (coefficients 1/k!, truncation below
+// 1e-19), turned by the quadrant n mod 4.  Uses a handful of registers, where
+// the library routine's wide-argument path needs about a hundred.
+__device__ __forceinline__ void qsincos(double x, double* sn, double* cs) {
+  const double n = rint(x * 0x1.45f306dc9c883p-1);
+  double r = fma(-n, 0x1.921fb54442d18p+0, x);
+  r = fma(-n, 0x1.1a62633145c07p-54, r);
+  r = fma(-n, -0x1.f1976b7ed8fbcp-110, r);
+  const double r2 = r * r;
+  double ps = 1.0 / 355687428096000.0;
+  ps = fma(ps, r2, -1.0 / 1307674368000.0);
+  ps = fma(ps, r2, 1.0 / 6227020800.0);
+  ps = fma(ps, r2, -1.0 / 39916800.0);
+  ps = fma(ps, r2, 1.0 / 362880.0);
+  ps = fma(ps, r2, -1.0 / 5040.0);
+  ps = fma(ps, r2, 1.0 / 120.0);
+  ps = fma(ps, r2, -1.0 / 6.0);
+  const double sr = fma(r * r2, ps, r);
+  double pc = -1.0 / 6402373705728000.0;
+  pc = fma(pc, r2, 1.0 / 20922789888000.0);
+  pc = fma(pc, r2, -1.0 / 87178291200.0);
+  pc = fma(pc, r2, 1.0 / 479001600.0);
+  pc = fma(pc, r2, -1.0 / 3628800.0);
+  pc = fma(pc, r2, 1.0 / 40320.0);
+  pc = fma(pc, r2, -1.0 / 720.0);
+  pc = fma(pc, r2, 1.0 / 24.0);
+  pc = fma(pc, r2, -0.5);
+  const double cr = fma(pc, r2, 1.0);
+  const int q = (int)((long long)n & 3);
+  const double sv = (q & 1) ? cr : sr, cv = (q & 1) ? sr : cr;
+  *sn = (q & 2) ? -sv : sv;
+  *cs = ((q + 1) & 2) ? -cv : cv;
+}
+
+enum { TK_U2 = 0, TK_U3 = 1, TK_B = 2, TK_V = 3 };
+
+// Element i of a 2- or 3-entry member array without a dynamic index (a
+// dynamic index into the kernel-argument struct would copy it to scratch).
+template <typename T>
+__device__ __forceinline__ T pick2(const T (&a)[2], int i) { return i ? a[1] : a[0]; }
+template <typename T>
+__device__ __forceinline__ T pick3(const T (&a)[3], int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+
+struct TabRef {
+  int row0, ng;
+};
+template <int KIND>
+__device__ __forceinline__ TabRef tabref(const DevSys& s, int tp) {
+  return TabRef{pick3(s.trow[KIND], tp), pick3(s.tng[KIND], tp)};
+}
+
 // Cubic spline on the uniform grid, value and two derivatives
 // (getf, reptation.f:4114-4125).
-__device__ __forceinline__ void spline3(const DevSys& s, int t, int ind, double rem, double& f,
+__device__ __forceinline__ void spline3(const DevSys& s, TabRef t, int ind, double rem, double& f,
                                         double& df, double& ddf) {
-  int i = ind < s.ngrid[t] ? (ind > 0 ? ind : 0) : s.ngrid[t];
-  double4 c = s.tab[(int64_t)t * s.tab_ld + i];
+  int i = ind < t.ng ? (ind > 0 ? ind : 0) : t.ng;
+  double4 c = s.tab[t.row0 + i];
   double rm = rem * s.drti;
   f = c.x + rm * (c.y + rm * (c.z + rm * c.w));
   df = (c.y + rm * (2.0 * c.z + rm * (3.0 * c.w))) * s.drti;
@@ -90,12 +153,12 @@ __device__ __forceinline__ void spline3(const DevSys& s, int t, int ind, double 
 }
 
 // Value only, times (1/r)^iexp (getf_0, reptation.f:4103-4112).
-__device__ __forceinline__ double spline0(const DevSys& s, int t, int ind, double rem, double byr) {
-  int i = ind < s.ngrid[t] ? (ind > 0 ? ind : 0) : s.ngrid[t];
-  double4 c = s.tab[(int64_t)t * s.tab_ld + i];
+__device__ __forceinline__ double spline0(const DevSys& s, TabRef t, int e, int ind, double rem,
+                                         double byr) {
+  int i = ind < t.ng ? (ind > 0 ? ind : 0) : t.ng;
+  double4 c = s.tab[t.row0 + i];
   double rm = rem * s.drti;
   double f = c.x + rm * (c.y + rm * (c.z + rm * c.w));
-  int e = s.iexp[t];
   double p = 1.0;
   int n = e < 0 ? -e : e;
   for (int k = 0; k < n; k++) p *= byr;

@@ -36,6 +36,8 @@ struct qmc_ctx {
   double* d_kvec = nullptr;
   double* d_vk = nullptr;
   double* d_gvec = nullptr;
+  double* d_kxyv = nullptr;  // distinct kx then ky components
+  int* d_kxy = nullptr;
   // workspace sized for ws_nw walkers
   int ws_nw = 0;
   double* d_ws = nullptr;
@@ -50,6 +52,8 @@ struct qmc_ctx {
   int kc = 16;
   size_t lds_pairs = 0, lds_det = 0, lds_cross = 0;
   int tpp_pairs = 0, tpp_det = 0;
+  bool use_gj = false;   // register Gauss-Jordan determinant path
+  size_t lds_fgrad = 0;
   // per-kernel event timing
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 5 events per recorded launch

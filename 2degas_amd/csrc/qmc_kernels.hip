@@ -18,9 +18,12 @@
 //             (kinetic, reptation.f:2729-2775; potential tail, 4088-4096).
 #include "qmc_dev.h"
 
+// k-vectors per thread in k_pairs (nk <= KMAXPT * QMC_BLOCK)
+#define KMAXPT 8
+
 // ---------------------------------------------------------------------------
 // k_pairs
-// LDS: xs[2N], Gs[2N], red[8]
+// LDS: xs[2N], Gs[2N], red[8], then kchunk x (nkxv + nkyv) phases
 __global__ __launch_bounds__(QMC_BLOCK) void k_pairs(DevSys s, WsLayout L,
                                                      const double* __restrict__ x,
                                                      double* __restrict__ ws,
@@ -55,7 +58,7 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_pairs(DevSys s, WsLayout L,
         PairGeo p = pair_geo(s, xi, yi, xs[2 * j], xs[2 * j + 1]);
         double f, df, ddf;
         if (s.has_u2) {
-          spline3(s, s.tu2[tp], p.ind, p.rem, f, df, ddf);
+          spline3(s, tabref<TK_U2>(s, tp), p.ind, p.rem, f, df, ddf);
           ltf += 0.5 * f;
           const double dtr = df * p.byr;
           gx += dtr * p.ux;
@@ -63,7 +66,7 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_pairs(DevSys s, WsLayout L,
           hh += ddf + dtr;
         }
         if (s.has_u3) {
-          spline3(s, s.tu3[tp], p.ind, p.rem, f, df, ddf);
+          spline3(s, tabref<TK_U3>(s, tp), p.ind, p.rem, f, df, ddf);
           Gx -= f * p.ux;
           Gy -= f * p.uy;
           // (xi r)^2 pair term of the three-body factor (reptation.f:2822-2832)
@@ -77,12 +80,13 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_pairs(DevSys s, WsLayout L,
           hh += dd3 + d3;
         }
         if (s.has_b) {
-          spline3(s, s.tub[tp], p.ind, p.rem, f, df, ddf);
+          spline3(s, tabref<TK_B>(s, tp), p.ind, p.rem, f, df, ddf);
           qx += f * p.ux;
           qy += f * p.uy;
         }
         if (s.has_v2) {
-          vv += 0.5 * spline0(s, s.tv2[tp], p.ind, p.rem, p.byr) * s.v2value[tp];
+          vv += 0.5 * spline0(s, tabref<TK_V>(s, tp), pick3(s.vexp, tp), p.ind, p.rem, p.byr) *
+                pick3(s.v2value, tp);
         }
       }
     }
@@ -129,7 +133,7 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_pairs(DevSys s, WsLayout L,
           const int tp = tpair(ti, ptype(s, j));
           PairGeo p = pair_geo(s, xi, yi, xs[2 * j], xs[2 * j + 1]);
           double t, dt, ddt;
-          spline3(s, s.tu3[tp], p.ind, p.rem, t, dt, ddt);
+          spline3(s, tabref<TK_U3>(s, tp), p.ind, p.rem, t, dt, ddt);
           const double c = dt * p.byr;
           const double m11 = t + c * p.ux * p.ux, m12 = c * p.ux * p.uy, m22 = t + c * p.uy * p.uy;
           const double dGx = Gix - Gs[2 * j], dGy = Giy - Gs[2 * j + 1];
@@ -165,35 +169,91 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_pairs(DevSys s, WsLayout L,
   double pk0 = 0.0, pk1 = 0.0, pkt = 0.0;
   if (s.nrhok) {
     double* pw = props + (int64_t)w * s.n_props;
-    for (int k = threadIdx.x; k < s.nk; k += QMC_BLOCK) {
-      const double kx = s.kvec[2 * k], ky = s.kvec[2 * k + 1];
-      double rc[2] = {0.0, 0.0}, rs[2] = {0.0, 0.0};
-      for (int t = 0; t < s.ntypes; t++) {
-        if (!s.irhok[t]) continue;
-        double c = 0.0, sn = 0.0;
-        const int i1 = s.ipf[t] + s.np[t];
-        for (int i = s.ipf[t]; i < i1; i++) {
+    const int kpt = (s.nk + QMC_BLOCK - 1) / QMC_BLOCK;
+    double rc0[KMAXPT], rc1[KMAXPT], rs0[KMAXPT], rs1[KMAXPT];
+#pragma unroll
+    for (int q = 0; q < KMAXPT; q++) rc0[q] = rc1[q] = rs0[q] = rs1[q] = 0.0;
+    if (s.nkxv > 0) {
+      // e^{i k.x} = e^{i kx x} e^{i ky y}: per particle chunk, the distinct
+      // x and y phases go to LDS, then each thread multiplies its k's pairs
+      double2* ph = (double2*)(sm + 4 * N + 8);
+      const int nv = s.nkxv + s.nkyv;
+      for (int c0 = 0; c0 < N; c0 += s.kchunk) {
+        const int cn = min(s.kchunk, N - c0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < cn * nv; e += QMC_BLOCK) {
+          const int pc = e / nv, v = e - pc * nv;
+          const int i = c0 + pc;
+          const double arg = v < s.nkxv ? s.kxv[v] * xs[2 * i] : s.kyv[v - s.nkxv] * xs[2 * i + 1];
           double sv, cv;
-          sincos(kx * xs[2 * i] + ky * xs[2 * i + 1], &sv, &cv);
-          c += cv;
-          sn += sv;
+          sincos(arg, &sv, &cv);
+          ph[e] = make_double2(cv, sv);
         }
-        rc[t] = c;
-        rs[t] = sn;
-        const int slot = s.jrhok + (s.irhok[t] - 1) * 2 * s.nk + 2 * k;
-        pw[slot] = c;
-        pw[slot + 1] = sn;
-      }
-      for (int it = 0; it < s.ntypes; it++)
-        for (int jt = 0; jt < s.ntypes; jt++) {
-          const int tp = tpair(it, jt);
-          if (k < s.nke[tp]) {
-            const double aux = s.vk[tp][k] * (rc[it] * rc[jt] + rs[it] * rs[jt]);
-            if (it == 0) pk0 += aux;
-            else pk1 += aux;
-            pkt += aux;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < KMAXPT; q++) {
+          const int k = threadIdx.x + q * QMC_BLOCK;
+          if (q >= kpt || k >= s.nk) continue;
+          const int code = s.kxy[k];
+          const int ix = code & 0xffff, iy = (code >> 16) + s.nkxv;
+          for (int pc = 0; pc < cn; pc++) {
+            const int t = ptype(s, c0 + pc);
+            const double2 a = ph[pc * nv + ix], bb = ph[pc * nv + iy];
+            const double cr = a.x * bb.x - a.y * bb.y, ci = a.x * bb.y + a.y * bb.x;
+            if (t == 0) { rc0[q] += cr; rs0[q] += ci; }
+            else { rc1[q] += cr; rs1[q] += ci; }
           }
         }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < KMAXPT; q++) {
+        const int k = threadIdx.x + q * QMC_BLOCK;
+        if (q >= kpt || k >= s.nk) continue;
+        const double kx = s.kvec[2 * k], ky = s.kvec[2 * k + 1];
+        for (int i = 0; i < N; i++) {
+          double sv, cv;
+          sincos(kx * xs[2 * i] + ky * xs[2 * i + 1], &sv, &cv);
+          if (ptype(s, i) == 0) { rc0[q] += cv; rs0[q] += sv; }
+          else { rc1[q] += cv; rs1[q] += sv; }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < KMAXPT; q++) {
+      const int k = threadIdx.x + q * QMC_BLOCK;
+      if (q >= kpt || k >= s.nk) continue;
+      if (!s.irhok[0]) rc0[q] = rs0[q] = 0.0;
+      if (s.ntypes < 2 || !s.irhok[1]) rc1[q] = rs1[q] = 0.0;
+      if (s.irhok[0]) {
+        const int slot = s.jrhok + (s.irhok[0] - 1) * 2 * s.nk + 2 * k;
+        pw[slot] = rc0[q];
+        pw[slot + 1] = rs0[q];
+      }
+      if (s.ntypes > 1 && s.irhok[1]) {
+        const int slot = s.jrhok + (s.irhok[1] - 1) * 2 * s.nk + 2 * k;
+        pw[slot] = rc1[q];
+        pw[slot + 1] = rs1[q];
+      }
+      // ordered type pairs (0,0), (0,1), (1,0), (1,1) (reptation.f:4034-4052)
+      if (k < s.nke[0]) {
+        const double aux = s.vk[0][k] * (rc0[q] * rc0[q] + rs0[q] * rs0[q]);
+        pk0 += aux;
+        pkt += aux;
+      }
+      if (s.ntypes > 1) {
+        if (k < s.nke[1]) {
+          const double aux = s.vk[1][k] * (rc0[q] * rc1[q] + rs0[q] * rs1[q]);
+          pk0 += aux;
+          pk1 += aux;
+          pkt += 2.0 * aux;
+        }
+        if (k < s.nke[2]) {
+          const double aux = s.vk[2][k] * (rc1[q] * rc1[q] + rs1[q] * rs1[q]);
+          pk1 += aux;
+          pkt += aux;
+        }
+      }
     }
   }
 
@@ -210,8 +270,8 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_pairs(DevSys s, WsLayout L,
     for (int it = 0; it < s.ntypes; it++)
       for (int jt = 0; jt < s.ntypes; jt++) {
         const int tp = tpair(it, jt);
-        if (s.nke[tp] != 0) {
-          const double a = s.vk0[tp] * s.npnorm;
+        if (pick3(s.nke, tp) != 0) {
+          const double a = pick3(s.vk0, tp) * s.npnorm;
           if (it == 0) c0 += a;
           else c1 += a;
           ct += a;
@@ -245,16 +305,16 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_det(DevSys s, WsLayout L,
   __shared__ zd s_pv, s_akk;
   __shared__ double red[QMC_BLOCK / 64];
   const int w = blockIdx.x, sp = blockIdx.y;
-  const int m = s.np[sp];
+  const int m = pick2(s.np, sp);
   if (m == 0) return;
-  const int N = s.nptot, p0 = s.ipf[sp];
+  const int N = s.nptot, p0 = pick2(s.ipf, sp);
   const int tid = threadIdx.x;
   zd* A = (zd*)sm;
   double* U = sm + 2 * m * m;
   double* wsw = ws + (int64_t)w * L.per_walker;
-  double* dsc = wsw + L.off_det[sp];
-  zd* Ag = (zd*)(Abuf + (int64_t)w * L.a_per_walker + L.a_off[sp]);  // row-major [k][j]
-  zd* Fg = (zd*)(Fbuf + (int64_t)w * L.f_per_walker + L.f_off[sp]);   // [beta][l][j]
+  double* dsc = wsw + pick2(L.off_det, sp);
+  zd* Ag = (zd*)(Abuf + (int64_t)w * L.a_per_walker + pick2(L.a_off, sp));  // row-major [k][j]
+  zd* Fg = (zd*)(Fbuf + (int64_t)w * L.f_per_walker + pick2(L.f_off, sp));   // [beta][l][j]
   const double* qw = wsw + L.off_q;
   if (tid == 0) s_sing = 0;
 
@@ -498,7 +558,7 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_det(DevSys s, WsLayout L,
     const int ip = i0 + grp;
     const bool act = ip < N;
     const int ti = act ? ptype(s, ip) : 0;
-    const bool on = act && s.iub_flag[sp][ti];
+    const bool on = act && (sp ? pick2(s.iub_flag[1], ti) : pick2(s.iub_flag[0], ti));
     const bool insp = ip >= p0 && ip < p0 + m;
     double zxr = 0, zxi = 0, zyr = 0, zyi = 0, hq = 0, hd = 0;
     double q11 = 0, q12 = 0, q22 = 0, bx = 0, by = 0;
@@ -506,8 +566,8 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_det(DevSys s, WsLayout L,
       const double xi = xs[2 * ip], yi = xs[2 * ip + 1];
       for (int c = lane; c < N; c += tpp) {
         if (c == ip) continue;
-        const int tb = s.tub[tpair(ti, ptype(s, c))];
-        if (tb < 0) continue;
+        const TabRef tb = tabref<TK_B>(s, tpair(ti, ptype(s, c)));
+        if (tb.row0 < 0) continue;
         PairGeo p = pair_geo(s, xi, yi, xs[2 * c], xs[2 * c + 1]);
         double eta, deta, ddeta;
         spline3(s, tb, p.ind, p.rem, eta, deta, ddeta);
@@ -570,7 +630,7 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_det(DevSys s, WsLayout L,
         }
         gdet[2 * ip] = -zxr;
         gdet[2 * ip + 1] = -zyr;
-        acc_g2 += s.hbs2m[ti] * (zxi * zxi + zyi * zyi);
+        acc_g2 += pick2(s.hbs2m, ti) * (zxi * zxi + zyi * zyi);
         if (ti == 0) acc_h0 -= hq + hd;
         else acc_h1 -= hq + hd;
       } else {
@@ -603,15 +663,15 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_cross(DevSys s, WsLayout L,
   extern __shared__ double sm[];
   __shared__ double red[QMC_BLOCK / 64];
   const int w = blockIdx.x, sp = blockIdx.y;
-  const int m = s.np[sp];
+  const int m = pick2(s.np, sp);
   if (m == 0) return;
-  const int N = s.nptot, p0 = s.ipf[sp];
+  const int N = s.nptot, p0 = pick2(s.ipf, sp);
   const int tid = threadIdx.x;
   double* wsw = ws + (int64_t)w * L.per_walker;
-  double* dsc = wsw + L.off_det[sp];
+  double* dsc = wsw + pick2(L.off_det, sp);
   if (dsc[DS_SING] != 0.0) return;
   const double* Qd = dsc + DS_N + 2 * N;
-  const zd* Fg = (const zd*)(Fbuf + (int64_t)w * L.f_per_walker + L.f_off[sp]);
+  const zd* Fg = (const zd*)(Fbuf + (int64_t)w * L.f_per_walker + pick2(L.f_off, sp));
   const int R = 2 * m;
   const int nT = (R + 7) >> 3;
   const int RP = nT * 8;
@@ -629,38 +689,16 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_cross(DevSys s, WsLayout L,
     TK = row + t;
   }
   const bool tact = tid < ntile;
-  double Mt[8][8];
-#pragma unroll
-  for (int a = 0; a < 8; a++)
-#pragma unroll
-    for (int b = 0; b < 8; b++) Mt[a][b] = 0.0;
-  if (tact) {
-#pragma unroll
-    for (int a = 0; a < 8; a++) {
-      const int r = TJ * 8 + a;
-      const int j = r >> 1, ra = r & 1;
-      if (j >= m) continue;
-#pragma unroll
-      for (int b = 0; b < 8; b++) {
-        const int c = TK * 8 + b;
-        const int k = c >> 1, cb = c & 1;
-        if (k >= m) continue;
-        const zd f1 = Fg[(ra * m + k) * m + j];
-        const zd f2 = Fg[(cb * m + j) * m + k];
-        Mt[a][b] = f1.x * f2.x - f1.y * f2.y;
-      }
-    }
-  }
   __syncthreads();
   double xc[2] = {0.0, 0.0};
   for (int t = 0; t < s.ntypes; t++) {
-    if (!s.iub_flag[sp][t] || s.np[t] == 0) continue;
+    if (!(sp ? pick2(s.iub_flag[1], t) : pick2(s.iub_flag[0], t)) || pick2(s.np, t) == 0) continue;
     double St[8][8];
 #pragma unroll
     for (int a = 0; a < 8; a++)
 #pragma unroll
       for (int b = 0; b < 8; b++) St[a][b] = 0.0;
-    const int ia = s.ipf[t], ib = s.ipf[t] + s.np[t];
+    const int ia = pick2(s.ipf, t), ib = pick2(s.ipf, t) + pick2(s.np, t);
     for (int c0 = ia; c0 < ib; c0 += XC_CP) {
       const int cn = min(XC_CP, ib - c0);
       // V chunk: rows (j,a), columns (ip,idim)
@@ -676,8 +714,8 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_cross(DevSys s, WsLayout L,
             v21 = v12;
             v22 = 1.0 + Qd[3 * j + 2];
           } else {
-            const int tb = s.tub[tpair(sp, ptype(s, ip))];
-            if (tb >= 0) {
+            const TabRef tb = tabref<TK_B>(s, tpair(sp, ptype(s, ip)));
+            if (tb.row0 >= 0) {
               PairGeo p = pair_geo(s, xs[2 * jp], xs[2 * jp + 1], xs[2 * ip], xs[2 * ip + 1]);
               double eta, deta, ddeta;
               spline3(s, tb, p.ind, p.rem, eta, deta, ddeta);
@@ -710,12 +748,25 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_cross(DevSys s, WsLayout L,
       }
       __syncthreads();
     }
+    // contract with M = Re(F_a(k,j) F_b(j,k)), formed from F (L2) here so
+    // that only the S tile occupies registers during the accumulation
     double acc = 0.0;
     if (tact) {
 #pragma unroll
-      for (int a = 0; a < 8; a++)
+      for (int a = 0; a < 8; a++) {
+        const int r = TJ * 8 + a;
+        const int j = r >> 1, ra = r & 1;
+        if (j >= m) continue;
 #pragma unroll
-        for (int b = 0; b < 8; b++) acc = fma(Mt[a][b], St[a][b], acc);
+        for (int b = 0; b < 8; b++) {
+          const int c = TK * 8 + b;
+          const int k = c >> 1, cb = c & 1;
+          if (k >= m) continue;
+          const zd f1 = Fg[(ra * m + k) * m + j];
+          const zd f2 = Fg[(cb * m + j) * m + k];
+          acc = fma(f1.x * f2.x - f1.y * f2.y, St[a][b], acc);
+        }
+      }
       if (TJ != TK) acc *= 2.0;
     }
     xc[t] = block_sum(acc, red);
@@ -744,8 +795,8 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_final(DevSys s, WsLayout L, doubl
   bool sing = false, guard = false;
   if (s.has_det) {
     for (int sp = 0; sp < s.ntypes; sp++) {
-      if (s.np[sp] == 0) continue;
-      const double* dsc = wsw + L.off_det[sp];
+      if (pick2(s.np, sp) == 0) continue;
+      const double* dsc = wsw + pick2(L.off_det, sp);
       ltf -= dsc[DS_LOGDET];
       if (dsc[DS_SING] != 0.0) sing = true;
     }
@@ -757,8 +808,8 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_final(DevSys s, WsLayout L, doubl
   double g2ph = 0.0;
   if (use_det) {
     for (int sp = 0; sp < s.ntypes; sp++) {
-      if (s.np[sp] == 0) continue;
-      const double* dsc = wsw + L.off_det[sp];
+      if (pick2(s.np, sp) == 0) continue;
+      const double* dsc = wsw + pick2(L.off_det, sp);
       h[0] += dsc[DS_H0] + dsc[DS_X0];
       h[1] += dsc[DS_H1] + dsc[DS_X1];
       g2ph += dsc[DS_G2];
@@ -770,8 +821,8 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_final(DevSys s, WsLayout L, doubl
     double gx = wsw[L.off_gj + 2 * i], gy = wsw[L.off_gj + 2 * i + 1];
     if (use_det) {
       for (int sp = 0; sp < s.ntypes; sp++) {
-        if (s.np[sp] == 0) continue;
-        const double* gd = wsw + L.off_det[sp] + DS_N;
+        if (pick2(s.np, sp) == 0) continue;
+        const double* gd = wsw + pick2(L.off_det, sp) + DS_N;
         gx += gd[2 * i];
         gy += gd[2 * i + 1];
       }
@@ -780,14 +831,15 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_final(DevSys s, WsLayout L, doubl
     const double x2 = gx * gx + gy * gy;
     if (s.adrift != 0.0) {
       // drift cutoff (kinetic, reptation.f:2747-2760)
-      const double vari = 1.0 / (2.0 * s.hbs2m[t] * s.delta);
+      const double vari = 1.0 / (2.0 * pick2(s.hbs2m, t) * s.delta);
       double gg = x2 * s.adrift * s.delta * s.delta * vari;
       gg = (-1.0 + sqrt(1.0 + 2.0 * gg)) / gg;
       ggs += gg;
       gx *= gg;
       gy *= gg;
     }
-    g2t[t] += x2;
+    if (t == 0) g2t[0] += x2;
+    else g2t[1] += x2;
     gw[2 * i] = gx;
     gw[2 * i + 1] = gy;
   }
@@ -797,7 +849,9 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_final(DevSys s, WsLayout L, doubl
   if (lane == 0) {
     double* p = props + (int64_t)w * s.n_props;
     double kin0 = 0.0;
-    for (int t = 0; t < s.ntypes; t++) {
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      if (t >= s.ntypes) break;
       const double kin = s.hbs2m[t] * (h[t] - g2t[t]);
       kin0 += kin;
       p[s.jkin[t]] = kin / s.np[t];
@@ -811,7 +865,9 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_final(DevSys s, WsLayout L, doubl
     const double g2v = grad2[w];
     p[s.jefp] = (s.ntheta != 0) ? g2v / s.npnorm : 0.0;
     p[s.jltf] = ltf;
-    for (int t = 0; t < s.ntypes; t++) p[s.jpot[t]] = sc[SC_P0 + t] / s.np[t];
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+      if (t < s.ntypes) p[s.jpot[t]] = sc[SC_P0 + t] / s.np[t];
     p[s.jpot0] = (sc[SC_PT] + s.v0) / s.npnorm;
     p[s.jetot] = p[s.jpot0] + p[s.jkin0] + p[s.jefp];
     p[s.je2] = p[s.jetot] * p[s.jetot];

@@ -34,9 +34,10 @@ __global__ void k_vmc_propose(DevSys s, int nw, const double* __restrict__ x_old
   double wo = 0.0;
   for (int t = 0; t < s.ntypes; t++) {
     const double var = t == 0 ? var0 : var1;
-    if (s.hbs2m[t] == 0.0) continue;
+    if (pick2(s.hbs2m, t) == 0.0) continue;
     double wt = 0.0;
-    for (int ip = s.ipf[t]; ip < s.ipf[t] + s.np[t]; ip++)
+    const int i0 = pick2(s.ipf, t), i1 = i0 + pick2(s.np, t);
+    for (int ip = i0; ip < i1; ip++)
       for (int d = 0; d < 2; d++) {
         double ua, ub;
         if (bm_order == 0) { ua = rannyu(st); ub = rannyu(st); }
@@ -77,7 +78,8 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_vmc_accept(
     for (int d = 0; d < 2; d++) {
       const double drift = -var * gn[2 * ip + d];
       const double chi = xo[2 * ip + d] - xn[2 * ip + d] - drift;
-      wt[t] = wt[t] - chi * chi;
+      if (t == 0) wt[0] = wt[0] - chi * chi;
+      else wt[1] = wt[1] - chi * chi;
     }
   }
   wt[0] = wave_sum(wt[0]);
@@ -91,8 +93,8 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_vmc_accept(
       p = 1.0;
     } else {
       double wnew = 0.0;
-      for (int t = 0; t < s.ntypes; t++)
-        if (s.hbs2m[t] != 0.0) wnew = wnew + wt[t] * (1.0 / (t == 0 ? var0 : var1));
+      if (s.hbs2m[0] != 0.0) wnew = wnew + wt[0] * (1.0 / var0);
+      if (s.ntypes > 1 && s.hbs2m[1] != 0.0) wnew = wnew + wt[1] * (1.0 / var1);
       p = (po[jltf] - pn[jltf]) * 2.0 + 0.5 * (wnew - wold[w]);
       p = fmin(0.0, fmax(-50.0, p));
       p = exp(p);
