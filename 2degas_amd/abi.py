@@ -109,6 +109,43 @@ class QmcWalkers(C.Structure):
     ]
 
 
+class QmcReptiles(C.Structure):
+    _fields_ = [
+        ("nr", C.c_int32),
+        ("ntau", C.c_int32),
+        ("nbuf", C.c_int32),
+        ("x", C.c_void_p),
+        ("g", C.c_void_p),
+        ("props", C.c_void_p),
+        ("age", C.c_void_p),
+        ("w", C.c_void_p),
+        ("ends", C.c_void_p),
+        ("rage", C.c_void_p),
+        ("pp_old", C.c_void_p),
+        ("grad2_ph", C.c_void_p),
+        ("rng", C.c_void_p),
+        ("blk", C.c_void_p),
+    ]
+
+
+class QmcRmcParams(C.Structure):
+    _fields_ = [
+        ("delta", C.c_double),
+        ("etrial", C.c_double),
+        ("mage", C.c_double),
+        ("mage_r", C.c_double),
+        ("ntauskip", C.c_int32),
+        ("mdelta", C.c_int32),
+        ("rejection", C.c_int32),
+        ("wpiu", C.c_int32),
+        ("gstorto", C.c_int32),
+        ("nodalaction", C.c_int32),
+        ("ecut", C.c_int32),
+        ("bm_order", C.c_int32),
+        ("value_ecut", C.c_double),
+    ]
+
+
 def limbs_to_state(limbs):
     """48-bit LCG state from the four 12-bit limbs savern writes."""
     l1, l2, l3, l4 = (int(v) for v in limbs)

@@ -1,0 +1,182 @@
+// qmc_cross.hip -- backflow Laplacian cross term on the FP64 matrix cores.
+//
+// For every spin s and particle type t with backflow between them,
+//   h_t += sum_{ip in t, idim} sum_{j,k} Re[(qa F)(qa F)]   (reptation.f:10241-10259)
+// which is the symmetric contraction sum_{r,q} M[r][q] S_t[r][q] over rows
+// r = (j, a) of the spin's particles (R = 2m <= 128 rows) with
+//   M[(j,a),(k,b)] = Re(F_a(k,j) F_b(j,k))            (symmetric),
+//   S_t = V_t V_t^T,  V_t[(j,a),(ip,idim)] = qa(a,idim,j,ip),
+// qa being the backflow Jacobian: 1 + Q_j on the particle's own block and
+// -(eta delta + eta' r u u) for a pair (reptation.f:10023-10058).
+//
+// One workgroup of 8 wavefronts per (walker, spin).  S_t is a 128 x 128
+// SYRK built from 16 x 16 tiles with v_mfma_f64_16x16x4f64: the 36 tiles of
+// the upper triangle go 9 to each pair of wavefronts, block rows g and 7-g
+// (8-g and g+1 tiles), split 5 + 4 between the pair.  V_t is staged in LDS 16
+// particles (32 columns) at a time, column-major so each MFMA operand is one
+// 8-byte LDS read per lane.  M is formed from F (L2) tile by tile when S_t is
+// complete, so only the S accumulators stay in registers through the chunks.
+#include "qmc_dev.h"
+
+typedef double xc_d4 __attribute__((ext_vector_type(4)));
+
+// Tile i (0..8) of the group for block row g: row g then row 7-g.
+__host__ __device__ constexpr int xc_tj(int g, int i) { return i < 8 - g ? g : 7 - g; }
+__host__ __device__ constexpr int xc_tk(int g, int i) { return i < 8 - g ? g + i : (7 - g) + (i - (8 - g)); }
+// wavefront half h takes tiles [xc_i0(h), xc_i0(h) + xc_nt(h))
+__host__ __device__ constexpr int xc_i0(int h) { return h ? 5 : 0; }
+__host__ __device__ constexpr int xc_nt(int h) { return h ? 4 : 5; }
+__host__ __device__ constexpr bool xc_needs(int g, int h, int b) {
+  for (int q = 0; q < xc_nt(h); q++)
+    if (xc_tj(g, xc_i0(h) + q) == b || xc_tk(g, xc_i0(h) + q) == b) return true;
+  return false;
+}
+
+#define XC_TPW 5  // most tiles per wavefront
+
+// Empty statement that takes the accumulator in registers: the MFMAs that
+// produce it are issued before this point.
+__device__ __forceinline__ void xc_pin(xc_d4& a) { __asm__ volatile("" : "+v"(a)); }
+
+// sum over the wavefront's tiles of M o S, M formed from F here (weight 2 off
+// the diagonal).  The accumulator of v_mfma_f64_16x16x4f64 holds, in lane l
+// and register v, element (row l/16 + 4v, column l%16) of the 16 x 16 tile.
+template <int G, int H>
+__device__ __forceinline__ double xc_contract(const xc_d4 (&acc)[XC_TPW], const zd* __restrict__ Fg,
+                                              int m, int lane, int nT) {
+  const int R = 2 * m;
+  double t = 0.0;
+#pragma unroll
+  for (int q = 0; q < xc_nt(H); q++) {
+    const int TJ = xc_tj(G, xc_i0(H) + q), TK = xc_tk(G, xc_i0(H) + q);
+    if (TK < nT) {
+      const int sc = 16 * TK + (lane & 15);
+      const int ks = sc >> 1, bs = sc & 1;
+      double tq = 0.0;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = 16 * TJ + (lane >> 4) + 4 * v;
+        if (r < R && sc < R) {
+          const int jr = r >> 1, ar = r & 1;
+          const zd f1 = Fg[(ar * m + ks) * m + jr];
+          const zd f2 = Fg[(bs * m + jr) * m + ks];
+          tq = fma(f1.x * f2.x - f1.y * f2.y, acc[q][v], tq);
+        }
+      }
+      t += TJ == TK ? tq : 2.0 * tq;
+    }
+    // one tile's F values in flight at a time
+    __asm__ volatile("" ::: "memory");
+  }
+  return t;
+}
+
+// Per-wavefront body for tile group (G, H): every chunk of V_t is staged by
+// the whole workgroup, then acc += V V^T on this wavefront's tiles.
+template <int G, int H>
+__device__ __forceinline__ void xc_body(const DevSys& s, double* __restrict__ dsc, const double* __restrict__ Qd,
+                                        const zd* __restrict__ Fg, const double* __restrict__ xs,
+                                        double* __restrict__ Vt, double* red, int sp, int m, int p0, int tid) {
+  const int lane = tid & 63;
+  const int nT = (2 * m + 15) >> 4;
+  const double* base = Vt + (lane >> 4) * XC_RS + (lane & 15);
+  double xc[2] = {0.0, 0.0};
+  for (int t = 0; t < s.ntypes; t++) {
+    if (!(sp ? pick2(s.iub_flag[1], t) : pick2(s.iub_flag[0], t)) || pick2(s.np, t) == 0) continue;
+    xc_d4 acc[XC_TPW];
+#pragma unroll
+    for (int q = 0; q < XC_TPW; q++) acc[q] = xc_d4{0.0, 0.0, 0.0, 0.0};
+    const int ia = pick2(s.ipf, t), ib = ia + pick2(s.np, t);
+    const TabRef tb = tabref<TK_B>(s, tpair(sp, t));
+    for (int c0 = ia; c0 < ib; c0 += XC_KC / 2) {
+      const int cn = min(XC_KC / 2, ib - c0);
+      // V chunk: 64 particle rows j (zero beyond m) x 16 particles (zero
+      // beyond cn); the 2 x 2 block of (j, ip) is symmetric
+      for (int e = tid; e < 64 * (XC_KC / 2); e += XC_THREADS) {
+        const int j = e >> 4, pc = e & 15;
+        double v11 = 0, v12 = 0, v22 = 0;
+        const int ip = c0 + pc;
+        if (j < m && pc < cn) {
+          const int jp = p0 + j;
+          if (jp == ip) {
+            v11 = 1.0 + Qd[3 * j];
+            v12 = Qd[3 * j + 1];
+            v22 = 1.0 + Qd[3 * j + 2];
+          } else if (tb.row0 >= 0) {
+            PairGeo pg = pair_geo(s, xs[2 * jp], xs[2 * jp + 1], xs[2 * ip], xs[2 * ip + 1]);
+            double eta, deta, ddeta;
+            spline3(s, tb, pg.ind, pg.rem, eta, deta, ddeta);
+            const double cb = deta * pg.byr;
+            v11 = -(eta + cb * pg.ux * pg.ux);
+            v12 = -(cb * pg.ux * pg.uy);
+            v22 = -(eta + cb * pg.uy * pg.uy);
+          }
+        }
+        Vt[(2 * pc) * XC_RS + 2 * j] = v11;
+        Vt[(2 * pc + 1) * XC_RS + 2 * j] = v12;
+        Vt[(2 * pc) * XC_RS + 2 * j + 1] = v12;
+        Vt[(2 * pc + 1) * XC_RS + 2 * j + 1] = v22;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < XC_KC / 4; ks++) {
+        // rows beyond 2m are zero in LDS, so tiles past the last block row
+        // add zeros and need no branch here
+        double blk[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+          if (xc_needs(G, H, b)) blk[b] = base[ks * 4 * XC_RS + 16 * b];
+#pragma unroll
+        for (int q = 0; q < xc_nt(H); q++) {
+          const int TJ = xc_tj(G, xc_i0(H) + q), TK = xc_tk(G, xc_i0(H) + q);
+          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(blk[TJ], blk[TK], acc[q], 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < xc_nt(H); q++) xc_pin(acc[q]);
+        // the next step's operands are read after this step's MFMAs issue
+        __asm__ volatile("" ::: "memory");
+      }
+      __syncthreads();
+    }
+    xc[t] = block_sum<XC_THREADS>(xc_contract<G, H>(acc, Fg, m, lane, nT), red);
+  }
+  if (tid == 0) {
+    dsc[DS_X0] = xc[0];
+    dsc[DS_X1] = xc[1];
+  }
+}
+
+// LDS: xs[2N], then Vt[XC_KC][XC_RS] (column = (particle, idim) of the
+// chunk, row = (j, a)).
+__global__ __launch_bounds__(XC_THREADS, 4) void k_cross(DevSys s, WsLayout L, const double* __restrict__ x,
+                                                         double* __restrict__ ws,
+                                                         const double* __restrict__ Fbuf) {
+  extern __shared__ double sm[];
+  __shared__ double red[XC_THREADS / 64];
+  const int w = blockIdx.x, sp = blockIdx.y;
+  const int m = pick2(s.np, sp);
+  if (m == 0) return;
+  const int N = s.nptot, p0 = pick2(s.ipf, sp);
+  const int tid = threadIdx.x;
+  const int w8 = __builtin_amdgcn_readfirstlane(tid >> 6);
+  double* wsw = ws + (int64_t)w * L.per_walker;
+  double* dsc = wsw + pick2(L.off_det, sp);
+  if (dsc[DS_SING] != 0.0) return;
+  const double* Qd = dsc + DS_N + 2 * N;
+  const zd* Fg = (const zd*)(Fbuf + (int64_t)w * L.f_per_walker + pick2(L.f_off, sp));
+  double* xs = sm;
+  double* Vt = sm + 2 * N;
+  const double* xw = x + (int64_t)w * 2 * N;
+  for (int k = tid; k < 2 * N; k += XC_THREADS) xs[k] = xw[k];
+  __syncthreads();
+  switch (w8) {
+    case 0: xc_body<0, 0>(s, dsc, Qd, Fg, xs, Vt, red, sp, m, p0, tid); break;
+    case 1: xc_body<1, 0>(s, dsc, Qd, Fg, xs, Vt, red, sp, m, p0, tid); break;
+    case 2: xc_body<2, 0>(s, dsc, Qd, Fg, xs, Vt, red, sp, m, p0, tid); break;
+    case 3: xc_body<3, 0>(s, dsc, Qd, Fg, xs, Vt, red, sp, m, p0, tid); break;
+    case 4: xc_body<0, 1>(s, dsc, Qd, Fg, xs, Vt, red, sp, m, p0, tid); break;
+    case 5: xc_body<1, 1>(s, dsc, Qd, Fg, xs, Vt, red, sp, m, p0, tid); break;
+    case 6: xc_body<2, 1>(s, dsc, Qd, Fg, xs, Vt, red, sp, m, p0, tid); break;
+    default: xc_body<3, 1>(s, dsc, Qd, Fg, xs, Vt, red, sp, m, p0, tid); break;
+  }
+}

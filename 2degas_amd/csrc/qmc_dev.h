@@ -10,6 +10,10 @@
 #define QMC_MAXTAB 20
 #define QMC_MAX_ORB 64
 #define QMC_BLOCK 256
+// k_cross: threads, V columns staged per chunk, LDS row stride (doubles)
+#define XC_THREADS 512
+#define XC_KC 32
+#define XC_RS 136
 
 // Type pair index of particles of types ti, tj (0: 1-1, 1: 1-2, 2: 2-2).
 __host__ __device__ inline int tpair(int ti, int tj) { return ti + tj; }
@@ -123,6 +127,15 @@ __device__ __forceinline__ void qsincos(double x, double* sn, double* cs) {
   *cs = ((q + 1) & 2) ? -cv : cv;
 }
 
+// x <- a x mod 2^48 with a = 34522712143931 (the limbs 502, 1521, 4071, 2107
+// of reptation.f:5349-5354); u = x / 2^48 exactly.  Unsigned 64-bit products
+// wrap mod 2^64, a multiple of 2^48, so the low 48 bits are exact.
+__device__ __forceinline__ double rannyu(uint64_t& x) {
+  const uint64_t a = 34522712143931ULL;
+  x = (a * x) & ((1ULL << 48) - 1ULL);
+  return (double)x * 0x1p-48;
+}
+
 enum { TK_U2 = 0, TK_U3 = 1, TK_B = 2, TK_V = 3 };
 
 // Element i of a 2- or 3-entry member array without a dynamic index (a
@@ -195,6 +208,8 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// Sum over a workgroup of NT threads (red holds NT / 64 doubles).
+template <int NT = QMC_BLOCK>
 __device__ __forceinline__ double block_sum(double v, double* red) {
   v = wave_sum(v);
   int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -203,6 +218,6 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   __syncthreads();
   double t = 0.0;
 #pragma unroll
-  for (int k = 0; k < QMC_BLOCK / 64; k++) t += red[k];
+  for (int k = 0; k < NT / 64; k++) t += red[k];
   return t;
 }

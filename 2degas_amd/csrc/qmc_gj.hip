@@ -247,23 +247,35 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
   double* dd = dl + 4 * m;
   const double* xw = x + (int64_t)w * 2 * N;
   for (int k = tid; k < 2 * N; k += QMC_BLOCK) xs[k] = xw[k];
-  for (int j = tid; j < m; j += QMC_BLOCK) {
-    const zd fx = Fg[(0 * m + j) * m + j], fy = Fg[(1 * m + j) * m + j];
-    dl[4 * j] = fx.x;
-    dl[4 * j + 1] = fx.y;
-    dl[4 * j + 2] = fy.x;
-    dl[4 * j + 3] = fy.y;
+  // diag(F_a F_b): four lanes per orbital j split the sum over k
+  {
+    const int j = tid >> 2, part = tid & 3;
     double sxx = 0, sxy = 0, syy = 0;
-    for (int k = 0; k < m; k++) {
-      const zd ax = Fg[(0 * m + j) * m + k], ay = Fg[(1 * m + j) * m + k];
-      const zd bx = Fg[(0 * m + k) * m + j], by = Fg[(1 * m + k) * m + j];
-      sxx += ax.x * bx.x - ax.y * bx.y;
-      sxy += ax.x * by.x - ax.y * by.y;
-      syy += ay.x * by.x - ay.y * by.y;
+    if (j < m) {
+      for (int k = part; k < m; k += 4) {
+        const zd ax = Fg[(0 * m + j) * m + k], ay = Fg[(1 * m + j) * m + k];
+        const zd bx = Fg[(0 * m + k) * m + j], by = Fg[(1 * m + k) * m + j];
+        sxx += ax.x * bx.x - ax.y * bx.y;
+        sxy += ax.x * by.x - ax.y * by.y;
+        syy += ay.x * by.x - ay.y * by.y;
+      }
     }
-    dd[3 * j] = sxx;
-    dd[3 * j + 1] = sxy;
-    dd[3 * j + 2] = syy;
+#pragma unroll
+    for (int o = 1; o < 4; o <<= 1) {
+      sxx += __shfl_xor(sxx, o, 64);
+      sxy += __shfl_xor(sxy, o, 64);
+      syy += __shfl_xor(syy, o, 64);
+    }
+    if (j < m && part == 0) {
+      const zd fx = Fg[(0 * m + j) * m + j], fy = Fg[(1 * m + j) * m + j];
+      dl[4 * j] = fx.x;
+      dl[4 * j + 1] = fx.y;
+      dl[4 * j + 2] = fy.x;
+      dl[4 * j + 3] = fy.y;
+      dd[3 * j] = sxx;
+      dd[3 * j + 1] = sxy;
+      dd[3 * j + 2] = syy;
+    }
   }
   __syncthreads();
 

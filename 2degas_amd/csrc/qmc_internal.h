@@ -62,11 +62,17 @@ struct qmc_ctx {
   int vmc_nw = 0;
   double *d_xn = nullptr, *d_gn = nullptr, *d_hn = nullptr, *d_pn = nullptr, *d_wold = nullptr;
   int32_t *d_stn = nullptr, *d_sticky = nullptr;
+  // reptation scratch for rmc_nr reptiles: move draws and per-move sums
+  int rmc_nr = 0;
+  int32_t* d_rmv = nullptr;
+  double* d_rsc = nullptr;
 };
 
 // Queue one batched evaluation (k_pairs, k_det, k_cross, k_final) on st.
 int launch_eval(qmc_ctx* c, int nw, const double* d_x, double* d_props, double* d_g, double* d_h,
                 double* d_g2, int32_t* d_status, hipStream_t st);
 void free_vmc(qmc_ctx* c);
+int ensure_vmc_scratch(qmc_ctx* c, int nw);
+void free_rmc(qmc_ctx* c);
 
 

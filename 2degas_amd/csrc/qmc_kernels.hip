@@ -11,9 +11,9 @@
 //             the quasi-coordinates, LU with LINPACK pivoting and inverse in
 //             LDS (zgefa/zgedi, reptation.f:9227-9472), F = A^-1 dA, drift
 //             and the diagonal Laplacian terms (reptation.f:10159-10240).
-//   k_cross   one workgroup per (walker, spin): the backflow Laplacian cross
-//             term sum_ip tr(C_ip^2) (reptation.f:10241-10259) as a
-//             symmetric contraction of S = V V^T against M = Re(F o F^T).
+//   k_cross   (qmc_cross.hip) one workgroup per (walker, spin): the backflow
+//             Laplacian cross term sum_ip tr(C_ip^2) (reptation.f:10241-10259)
+//             as a symmetric contraction of S = V V^T against M = Re(F o F^T).
 //   k_final   kinetic and potential energies and the property slots
 //             (kinetic, reptation.f:2729-2775; potential tail, 4088-4096).
 #include "qmc_dev.h"
@@ -646,134 +646,6 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_det(DevSys s, WsLayout L,
     dsc[DS_H0] = acc_h0;
     dsc[DS_H1] = acc_h1;
     dsc[DS_G2] = acc_g2;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// k_cross: h_t += sum_{ip in t, idim} sum_{j,k} Re[(qa F)(qa F)] written as
-// sum_{rows r,s} M[r][s] S_t[r][s], rows r = (j, a) of the spin's particles,
-// M[(j,a),(k,b)] = Re(F_a(k,j) F_b(j,k)) (symmetric),
-// S_t = V_t V_t^T with V[(j,a),(ip,idim)] = qa(a,idim,j,ip).
-// Upper-triangle 8x8 tiles, one per thread; V staged 8 particles at a time.
-#define XC_CP 8
-__global__ __launch_bounds__(QMC_BLOCK) void k_cross(DevSys s, WsLayout L,
-                                                     const double* __restrict__ x,
-                                                     double* __restrict__ ws,
-                                                     const double* __restrict__ Fbuf) {
-  extern __shared__ double sm[];
-  __shared__ double red[QMC_BLOCK / 64];
-  const int w = blockIdx.x, sp = blockIdx.y;
-  const int m = pick2(s.np, sp);
-  if (m == 0) return;
-  const int N = s.nptot, p0 = pick2(s.ipf, sp);
-  const int tid = threadIdx.x;
-  double* wsw = ws + (int64_t)w * L.per_walker;
-  double* dsc = wsw + pick2(L.off_det, sp);
-  if (dsc[DS_SING] != 0.0) return;
-  const double* Qd = dsc + DS_N + 2 * N;
-  const zd* Fg = (const zd*)(Fbuf + (int64_t)w * L.f_per_walker + pick2(L.f_off, sp));
-  const int R = 2 * m;
-  const int nT = (R + 7) >> 3;
-  const int RP = nT * 8;
-  double* xs = sm;                 // [2N]
-  double* Vc = sm + 2 * N;         // [RP][2*XC_CP]
-  const double* xw = x + (int64_t)w * 2 * N;
-  for (int k = tid; k < 2 * N; k += QMC_BLOCK) xs[k] = xw[k];
-  // tile of this thread
-  const int ntile = nT * (nT + 1) / 2;
-  int TJ = 0, TK = 0;
-  {
-    int t = tid, row = 0;
-    while (row < nT && t >= nT - row) { t -= nT - row; row++; }
-    TJ = row;
-    TK = row + t;
-  }
-  const bool tact = tid < ntile;
-  __syncthreads();
-  double xc[2] = {0.0, 0.0};
-  for (int t = 0; t < s.ntypes; t++) {
-    if (!(sp ? pick2(s.iub_flag[1], t) : pick2(s.iub_flag[0], t)) || pick2(s.np, t) == 0) continue;
-    double St[8][8];
-#pragma unroll
-    for (int a = 0; a < 8; a++)
-#pragma unroll
-      for (int b = 0; b < 8; b++) St[a][b] = 0.0;
-    const int ia = pick2(s.ipf, t), ib = pick2(s.ipf, t) + pick2(s.np, t);
-    for (int c0 = ia; c0 < ib; c0 += XC_CP) {
-      const int cn = min(XC_CP, ib - c0);
-      // V chunk: rows (j,a), columns (ip,idim)
-      for (int e = tid; e < RP * XC_CP / 2; e += QMC_BLOCK) {
-        const int j = e / XC_CP, pc = e - j * XC_CP;
-        double v11 = 0, v12 = 0, v21 = 0, v22 = 0;  // [a][idim]
-        const int ip = c0 + pc;
-        if (j < m && pc < cn) {
-          const int jp = p0 + j;
-          if (jp == ip) {
-            v11 = 1.0 + Qd[3 * j];
-            v12 = Qd[3 * j + 1];
-            v21 = v12;
-            v22 = 1.0 + Qd[3 * j + 2];
-          } else {
-            const TabRef tb = tabref<TK_B>(s, tpair(sp, ptype(s, ip)));
-            if (tb.row0 >= 0) {
-              PairGeo p = pair_geo(s, xs[2 * jp], xs[2 * jp + 1], xs[2 * ip], xs[2 * ip + 1]);
-              double eta, deta, ddeta;
-              spline3(s, tb, p.ind, p.rem, eta, deta, ddeta);
-              const double cb = deta * p.byr;
-              v11 = -(eta + cb * p.ux * p.ux);
-              v12 = -(cb * p.ux * p.uy);
-              v21 = v12;
-              v22 = -(eta + cb * p.uy * p.uy);
-            }
-          }
-        }
-        Vc[(2 * j) * (2 * XC_CP) + 2 * pc] = v11;
-        Vc[(2 * j) * (2 * XC_CP) + 2 * pc + 1] = v12;
-        Vc[(2 * j + 1) * (2 * XC_CP) + 2 * pc] = v21;
-        Vc[(2 * j + 1) * (2 * XC_CP) + 2 * pc + 1] = v22;
-      }
-      __syncthreads();
-      if (tact) {
-        for (int col = 0; col < 2 * cn; col++) {
-          double va[8], vb[8];
-#pragma unroll
-          for (int a = 0; a < 8; a++) va[a] = Vc[(TJ * 8 + a) * (2 * XC_CP) + col];
-#pragma unroll
-          for (int b = 0; b < 8; b++) vb[b] = Vc[(TK * 8 + b) * (2 * XC_CP) + col];
-#pragma unroll
-          for (int a = 0; a < 8; a++)
-#pragma unroll
-            for (int b = 0; b < 8; b++) St[a][b] = fma(va[a], vb[b], St[a][b]);
-        }
-      }
-      __syncthreads();
-    }
-    // contract with M = Re(F_a(k,j) F_b(j,k)), formed from F (L2) here so
-    // that only the S tile occupies registers during the accumulation
-    double acc = 0.0;
-    if (tact) {
-#pragma unroll
-      for (int a = 0; a < 8; a++) {
-        const int r = TJ * 8 + a;
-        const int j = r >> 1, ra = r & 1;
-        if (j >= m) continue;
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-          const int c = TK * 8 + b;
-          const int k = c >> 1, cb = c & 1;
-          if (k >= m) continue;
-          const zd f1 = Fg[(ra * m + k) * m + j];
-          const zd f2 = Fg[(cb * m + j) * m + k];
-          acc = fma(f1.x * f2.x - f1.y * f2.y, St[a][b], acc);
-        }
-      }
-      if (TJ != TK) acc *= 2.0;
-    }
-    xc[t] = block_sum(acc, red);
-  }
-  if (tid == 0) {
-    dsc[DS_X0] = xc[0];
-    dsc[DS_X1] = xc[1];
   }
 }
 

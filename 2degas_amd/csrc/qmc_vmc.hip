@@ -7,15 +7,6 @@
 // reptation.f:37-39).
 #include "qmc_internal.h"
 
-// x <- a x mod 2^48 with a = 34522712143931 (the limbs 502, 1521, 4071, 2107
-// of reptation.f:5349-5354); u = x / 2^48 exactly.  Unsigned 64-bit products
-// wrap mod 2^64, a multiple of 2^48, so the low 48 bits are exact.
-__device__ __forceinline__ double rannyu(uint64_t& x) {
-  const uint64_t a = 34522712143931ULL;
-  x = (a * x) & ((1ULL << 48) - 1ULL);
-  return (double)x * 0x1p-48;
-}
-
 // move, proposal half (reptation.f:4630-4646): for every moving type,
 // particle and dimension chi = cos(pi u1) sqrt(-2 var log u2),
 // x_new = chi - var g_old + x_old, w = -sum chi^2 / var.
@@ -160,7 +151,8 @@ void free_vmc(qmc_ctx* c) {
   c->vmc_nw = 0;
 }
 
-static int ensure_vmc(qmc_ctx* c, int nw) {
+// Proposal and status buffers for nw walkers (or reptiles).
+int ensure_vmc_scratch(qmc_ctx* c, int nw) {
   if (nw <= c->vmc_nw) return QMC_OK;
   free_vmc(c);
   const size_t N = c->s.nptot, NP = c->info.n_props;
@@ -172,6 +164,8 @@ static int ensure_vmc(qmc_ctx* c, int nw) {
   HIPCHK(hipMalloc(&c->d_stn, sizeof(int32_t) * nw));
   HIPCHK(hipMalloc(&c->d_sticky, sizeof(int32_t) * nw));
   HIPCHK(hipMemset(c->d_pn, 0, sizeof(double) * NP * nw));
+  // a sweep may run without a start call (state loaded by the caller)
+  HIPCHK(hipMemset(c->d_sticky, 0, sizeof(int32_t) * nw));
   c->vmc_nw = nw;
   return QMC_OK;
 }
@@ -189,7 +183,7 @@ extern "C" int qmc_vmc_start(qmc_ctx* c, const qmc_walkers* w, void* stream) {
   if (w->nw == 0) return QMC_OK;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = (hipStream_t)stream;
-  rc = ensure_vmc(c, w->nw);
+  rc = ensure_vmc_scratch(c, w->nw);
   if (rc) return rc;
   rc = launch_eval(c, w->nw, w->x, w->props, w->g, w->h, w->grad2_ph, c->d_stn, st);
   if (rc) return rc;
@@ -208,7 +202,7 @@ extern "C" int qmc_vmc_sweep(qmc_ctx* c, const qmc_walkers* w, int nsteps, doubl
   if (!(delta > 0.0)) return arg_fail("delta must be positive");
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = (hipStream_t)stream;
-  rc = ensure_vmc(c, w->nw);
+  rc = ensure_vmc_scratch(c, w->nw);
   if (rc) return rc;
   const DevSys& s = c->s;
   // sigma (reptation.f:855-870)

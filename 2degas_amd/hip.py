@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(HERE, "lib2degas_hip.so")
 EXPORTS = ("qmc_ctx_create", "qmc_ctx_destroy", "qmc_ctx_info", "qmc_set_twist",
            "qmc_twist_gvec", "qmc_eval_batch", "qmc_eval_batch_device", "qmc_strerror",
            "qmc_last_error", "qmc_flops_per_eval", "qmc_timing", "qmc_timing_read",
-           "qmc_vmc_start", "qmc_vmc_sweep", "qmc_vmc_status")
+           "qmc_vmc_start", "qmc_vmc_sweep", "qmc_vmc_status", "qmc_rmc_start",
+           "qmc_rmc_sweep")
 
 _lib = None
 
@@ -64,6 +65,9 @@ def lib():
         L.qmc_vmc_sweep.argtypes = [vp, C.POINTER(abi.QmcWalkers), C.c_int, C.c_double,
                                     C.c_double, C.c_int, vp, vp, vp]
         L.qmc_vmc_status.argtypes = [vp, C.c_int, ip]
+        L.qmc_rmc_start.argtypes = [vp, C.POINTER(abi.QmcReptiles), C.POINTER(abi.QmcRmcParams), vp]
+        L.qmc_rmc_sweep.argtypes = [vp, C.POINTER(abi.QmcReptiles), C.POINTER(abi.QmcRmcParams),
+                                    C.c_int, vp, vp]
         L.qmc_timing_read.argtypes = [vp, dp, C.POINTER(C.c_int)]
         _lib = L
     return _lib
@@ -221,6 +225,96 @@ class Walkers:
     def status(self):
         st = np.zeros(self.nw, np.int32)
         rc = lib().qmc_vmc_status(self.ctx.h, self.nw, abi.iptr(st))
+        if rc:
+            raise QmcError(rc, "qmc_vmc_status")
+        return st
+
+
+# fixed fields of one reptation trace row before the per-bead probabilities
+RMC_TRACE_FIELDS = ("idir", "ndelta", "wnew", "age", "e", "ltf", "kin", "spare")
+
+
+def rmc_params(delta, etrial, mage, mage_r, ntauskip, mdelta, driver=None, bm_order=0):
+    """qmc_rmc_params from the rmc record and the deck's driver keywords
+    (rejection, wpiu, gstorto, nodalaction, ecut; reptation.f:149-170)."""
+    driver = driver or {}
+    p = abi.QmcRmcParams()
+    p.delta, p.etrial, p.mage, p.mage_r = delta, etrial, mage, mage_r
+    p.ntauskip, p.mdelta = ntauskip, mdelta
+    p.rejection = int(driver.get("rejection", ["1"])[0])
+    p.wpiu = int(driver.get("wpiu", ["0"])[0])
+    p.gstorto = int(driver.get("gstorto", ["0"])[0])
+    p.nodalaction = int(driver.get("nodalaction", ["0"])[0])
+    ec = driver.get("ecut", ["0"])
+    p.ecut = int(ec[0])
+    p.value_ecut = float(ec[1].replace("d", "e").replace("D", "E")) if p.ecut and len(ec) > 1 else 0.0
+    p.bm_order = bm_order
+    return p
+
+
+class Reptiles:
+    """Device reptile state of a batched reptation run (torch tensors on
+    cuda).  Ring slots and path ends are 0-based."""
+
+    def __init__(self, ctx, nr, ntau, mdelta, rng_states, block_sums=True):
+        import torch
+        self.ctx = ctx
+        dev = torch.device("cuda", 0)
+        N = ctx.deck.nptot
+        NP = ctx.info["n_props"]
+        f64 = torch.float64
+        self.nr, self.ntau, self.mdelta = nr, ntau, mdelta
+        nbuf = ntau + mdelta
+        self.nbuf = nbuf
+        self.x = torch.zeros((nr, nbuf, N, 2), dtype=f64, device=dev)
+        self.g = torch.zeros((nr, nbuf, N, 2), dtype=f64, device=dev)
+        self.props = torch.zeros((nr, nbuf, NP), dtype=f64, device=dev)
+        self.age = torch.zeros((nr, nbuf), dtype=f64, device=dev)
+        self.w = torch.zeros((nr, nbuf, 5), dtype=f64, device=dev)
+        self.ends = torch.zeros((nr, 2), dtype=torch.int32, device=dev)
+        self.rage = torch.zeros(nr, dtype=f64, device=dev)
+        self.pp_old = torch.zeros((nr, NP), dtype=f64, device=dev)
+        self.grad2 = torch.zeros(nr, dtype=f64, device=dev)
+        self.rng = torch.tensor(np.asarray(rng_states, dtype=np.int64), device=dev)
+        self.blk = torch.zeros((nr, NP), dtype=f64, device=dev) if block_sums else None
+        s = abi.QmcReptiles()
+        s.nr, s.ntau, s.nbuf = nr, ntau, nbuf
+        for name in ("x", "g", "props", "age", "w", "ends", "rage", "pp_old", "rng", "blk"):
+            t = getattr(self, name)
+            setattr(s, name, t.data_ptr() if t is not None else None)
+        s.grad2_ph = self.grad2.data_ptr()
+        self.s = s
+
+    def _stream(self, stream):
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream()
+        return C.c_void_p(st.cuda_stream)
+
+    def start(self, beads, params, stream=None):
+        """beads: [nr][ntau][N][2] initial paths (read_path)."""
+        import torch
+        self.x[:, :self.ntau] = torch.as_tensor(np.asarray(beads, dtype=np.float64),
+                                                device=self.x.device)
+        rc = lib().qmc_rmc_start(self.ctx.h, C.byref(self.s), C.byref(params), self._stream(stream))
+        if rc:
+            raise QmcError(rc, "qmc_rmc_start")
+
+    def sweep(self, nsteps, params, trace=False, stream=None):
+        import torch
+        tr = None
+        if trace:
+            tr = torch.zeros((nsteps, self.nr, len(RMC_TRACE_FIELDS) + self.mdelta),
+                             dtype=torch.float64, device=self.x.device)
+        rc = lib().qmc_rmc_sweep(self.ctx.h, C.byref(self.s), C.byref(params), nsteps,
+                                 C.c_void_p(tr.data_ptr()) if trace else None, self._stream(stream))
+        if rc:
+            raise QmcError(rc, "qmc_rmc_sweep")
+        return tr
+
+    def status(self):
+        """Sticky evaluation status per reptile (as Walkers.status)."""
+        st = np.zeros(self.nr, np.int32)
+        rc = lib().qmc_vmc_status(self.ctx.h, self.nr, abi.iptr(st))
         if rc:
             raise QmcError(rc, "qmc_vmc_status")
         return st

@@ -87,6 +87,9 @@ def main():
     ap.add_argument("--shells", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 1 GiB device-to-device copies: the known-size "
+                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every walker batch is independent, and this build "
@@ -94,6 +97,14 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "copies": 3}))
+        return
     hip = importlib.import_module("2degas_amd.hip")
     deck = importlib.import_module("2degas_amd.deck")
     dg = importlib.import_module("2degas_amd.deckgen")

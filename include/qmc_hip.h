@@ -201,6 +201,79 @@ int qmc_vmc_sweep(qmc_ctx* ctx, const qmc_walkers* w, int nsteps, double delta,
 int qmc_vmc_status(qmc_ctx* ctx, int nw, int32_t* sticky);
 
 /*
+ * Reptation QMC (the reference's rmc driver, reptation.f:1286-1313).  A
+ * reptile is a path of ntau beads held in a ring of nbuf = ntau + mdelta
+ * slots (the reference's x_stack/g_stack/p_stack/age_stack/w_stack with
+ * n_buffer slots, read_path at reptation.f:1766-1802).  Ring slots and the
+ * path ends are 0-based here (the reference's jfirst - 1, jlast - 1).  The
+ * bead sign s_stack is not stored: s_new is 1 on this path
+ * (reptation.f:3057), so the fixed-node rejection of reptation.f:1985 never
+ * fires.  All pointers are device pointers.
+ */
+typedef struct qmc_reptiles {
+  int32_t nr;       /* reptiles                                              */
+  int32_t ntau;     /* beads per path                                        */
+  int32_t nbuf;     /* ring slots, ntau + mdelta                             */
+  double* x;        /* [nr][nbuf][N][2]      bead positions (x_stack)        */
+  double* g;        /* [nr][nbuf][N][2]      bead drifts (g_stack)           */
+  double* props;    /* [nr][nbuf][n_props]   bead properties (p_stack)       */
+  double* age;      /* [nr][nbuf]            bead ages (age_stack)           */
+  double* w;        /* [nr][nbuf][5]         link weights (w_stack: jbra,
+                       jdtp, jitp, jlng, jnds)                               */
+  int32_t* ends;    /* [nr][2]               jfirst, jlast                   */
+  double* rage;     /* [nr]                  reptile age (age)               */
+  double* pp_old;   /* [nr][n_props]         path averages (p_p_old)         */
+  double* grad2_ph; /* [nr]                  fixed-phase term, carried from
+                       one evaluation to the next as the COMMON value is     */
+  int64_t* rng;     /* [nr]                  LCG state                       */
+  double* blk;      /* [nr][n_props]         block sums of p_old, or NULL    */
+} qmc_reptiles;
+
+/* Driver settings of the rmc record and the deck keywords that shape the
+ * move (reptation.f:795-805, 46-53, 149-170). */
+typedef struct qmc_rmc_params {
+  double delta;      /* time step (var = 2 hbs2m delta)                      */
+  double etrial;     /* trial energy in the branching weight                 */
+  double mage;       /* bead age that forces a Langevin acceptance           */
+  double mage_r;     /* reptile age that forces a reptation acceptance       */
+  int32_t ntauskip;  /* beads skipped at each end by path_props              */
+  int32_t mdelta;    /* largest number of beads moved at once                */
+  int32_t rejection; /* 0: Langevin steps are always accepted                */
+  int32_t wpiu;      /* propagator options of reptation_move                 */
+  int32_t gstorto;
+  int32_t nodalaction;
+  int32_t ecut;      /* fgg: e when 0, value_ecut + (e - value_ecut) gg      */
+  int32_t bm_order;  /* Box-Muller draw order, as for qmc_vmc_sweep         */
+  double value_ecut;
+} qmc_rmc_params;
+
+/*
+ * Start reptiles from the beads already in slots 0..ntau-1 of r->x (read_path,
+ * reptation.f:1766-1802): each bead is evaluated in order and accepted as
+ * read_conf does (age 0, p(jacc) = 1), the links get their weights and
+ * Langevin probabilities (get_w_stack, reptation.f:1804-1851), the path
+ * averages are formed (path_props, reptation.f:1556-1577) and the ends set to
+ * 0 and ntau-1.
+ */
+int qmc_rmc_start(qmc_ctx* ctx, const qmc_reptiles* r, const qmc_rmc_params* prm, void* stream);
+
+/*
+ * nsteps reptation moves for every reptile: reptation_move
+ * (reptation.f:1897-2113: direction and number of beads drawn, each bead a
+ * drift-diffusion proposal from the path end, evaluated and accepted by the
+ * Langevin test, then the path weight of the move) and
+ * reptation_metropolis_test (reptation.f:1865-1895), then averages(2)
+ * (blk += p_old) when r->blk is not NULL.  The random numbers of reptile k
+ * are those of MPI rank k of the reference.  trace (optional, device,
+ * [nsteps][nr][8 + mdelta]) receives per move: idir, ndelta, path weight,
+ * reptile age after the test, the path averages of the local energy,
+ * -log psi and the kinetic energy, spare, then the Langevin probability of
+ * each bead moved (-1 past ndelta).
+ */
+int qmc_rmc_sweep(qmc_ctx* ctx, const qmc_reptiles* r, const qmc_rmc_params* prm, int nsteps,
+                  double* trace, void* stream);
+
+/*
  * Per-kernel timing with HIP events recorded on the launch stream around each
  * kernel of every evaluation while enabled (no counterpart in the reference,
  * which has no timers).  qmc_timing(ctx, 1) resets and enables,

@@ -134,9 +134,12 @@ def have_ref():
 
 
 def run_refeval(deckdir, workdir, mode, x, twist=(1, 1), delta=0.0, nsteps=None,
-                seed=None):
+                seed=None, rmc=None, deck=None):
     """Run the reference harness on configurations x ([n][nptot][2]) in a
-    copy of the deck; returns the raw output bytes."""
+    copy of the deck; returns the raw output bytes.  Mode 4 (reptation)
+    takes rmc = dict(ntau, ntauskip, mdelta, etrial, mage, mage_r, every)
+    and the deck (for its per-type x_file names): x then holds the ntau
+    initial beads, written as <x_file>.0 the way read_conf reads them."""
     import shutil
     os.makedirs(workdir, exist_ok=True)
     for fn in os.listdir(deckdir):
@@ -148,6 +151,19 @@ def run_refeval(deckdir, workdir, mode, x, twist=(1, 1), delta=0.0, nsteps=None,
     n = nsteps if nsteps is not None else x.shape[0]
     with open(os.path.join(workdir, "refeval.ctl"), "w") as f:
         f.write(f"{mode}\n{twist[0]} {twist[1]}\n{n}\n{delta!r}\ncfg.bin\nout.bin\n")
+        if rmc is not None:
+            f.write("{ntau} {ntauskip} {mdelta} {etrial!r} {mage!r} {mage_r!r} {every}\n"
+                    .format(**{k: (float(v) if k in ("etrial", "mage", "mage_r") else int(v))
+                               for k, v in rmc.items()}))
+    if rmc is not None:
+        p0 = 0
+        for t in range(deck.ntypes):
+            npt = deck.np[t]
+            with open(os.path.join(workdir, deck.x_file[t] + ".0"), "w") as f:
+                for b in range(int(rmc["ntau"])):
+                    for ip in range(p0, p0 + npt):
+                        f.write(f"{float(x[b, ip, 0])!r} {float(x[b, ip, 1])!r}\n")
+            p0 += npt
     r = subprocess.run([os.path.join(REF_DIR, "refeval")], cwd=workdir,
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     if r.returncode != 0:
@@ -176,6 +192,52 @@ def parse_eval(raw, nptot):
     out["h"] = d[:, o:o + nt]; o += nt
     out["grad2_ph"] = d[:, o]; o += 1
     out["rhok"] = d[:, o:o + nr * 2 * nk]
+    return out
+
+
+def parse_rmc(raw, nptot, nsteps, every, mdelta):
+    """Mode 4 stream: per checkpoint the full reptile ring (1-based bead
+    indices as the reference keeps them), per move idir, ndelta, path weight,
+    bead acceptance probabilities, and after the test age and the path
+    averages of E, -log psi and the kinetic energy."""
+    o = 20
+    nps, nbuf, nprops = [int(v) for v in np.frombuffer(raw[o:o + 12], dtype=np.int32)]
+    o += 12
+
+    def take(n, dt=np.float64):
+        nonlocal o
+        sz = np.dtype(dt).itemsize * n
+        a = np.frombuffer(raw[o:o + sz], dtype=dt).copy()
+        o += sz
+        return a
+
+    ck = {k: [] for k in ("seed", "jfirst", "jlast", "age", "grad2_ph", "x", "g", "p",
+                          "age_stack", "w", "s", "pp_old")}
+    mv = {k: [] for k in ("idir", "ndelta", "wnew", "pbead", "age", "e", "ltf", "kin")}
+    for s in range(nsteps):
+        if s % every == 0:
+            h = take(6, np.int32)
+            ck["seed"].append(h[:4]); ck["jfirst"].append(h[4]); ck["jlast"].append(h[5])
+            a = take(2)
+            ck["age"].append(a[0]); ck["grad2_ph"].append(a[1])
+            ck["x"].append(take(nbuf * nptot * 2).reshape(nbuf, nptot, 2))
+            ck["g"].append(take(nbuf * nptot * 2).reshape(nbuf, nptot, 2))
+            ck["p"].append(take(nbuf * nps).reshape(nbuf, nps))
+            ck["age_stack"].append(take(nbuf))
+            ck["w"].append(take(5 * nbuf).reshape(5, nbuf).T.copy())
+            ck["s"].append(take(nbuf))
+            ck["pp_old"].append(take(nprops))
+        h = take(2, np.int32)
+        mv["idir"].append(h[0]); mv["ndelta"].append(h[1])
+        r = take(1 + mdelta)
+        mv["wnew"].append(r[0]); mv["pbead"].append(r[1:])
+        r = take(4)
+        mv["age"].append(r[0]); mv["e"].append(r[1]); mv["ltf"].append(r[2]); mv["kin"].append(r[3])
+    fin = take(6, np.int32)
+    out = {"ck_" + k: np.array(v) for k, v in ck.items()}
+    out.update({k: np.array(v) for k, v in mv.items()})
+    out.update(nps=nps, nbuf=nbuf, nprops=nprops, seed_final=fin[:4], jfirst_final=fin[4],
+               jlast_final=fin[5])
     return out
 
 

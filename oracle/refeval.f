@@ -15,18 +15,28 @@ c   delta                time step of the move (mode 2)
 c   cfgfile              raw float64 positions, [ncfg][nptot][2]
 c   outfile              raw output stream (layout below)
 c
+c Mode 4 only, line 7:
+c   ntau ntauskip mdelta etrial mage mage_r every
+c
 c Mode 1 follows read_conf (reptation.f:2452-2502) without the stack:
 c x_new <- config; compute_properties(1) (reptation.f:2661).
 c Mode 2 follows vmc (reptation.f:973-1017): one initial
 c compute_properties + metropolis_test(1) as read_conf does, then
 c move (reptation.f:4617) + metropolis_test (reptation.f:2622).
+c Mode 4 follows rmc (reptation.f:1286-1313): read_path (reptation.f:
+c 1766-1802, which reads the ntau initial beads from <x_file>.0 per
+c type), then reptation_move (reptation.f:1897) +
+c reptation_metropolis_test (reptation.f:1865) per move.  Every `every`
+c moves the whole reptile
+c state is dumped before the move.
       implicit none
       include 'ewald.h'
       integer mode,ncfg,icfg,ip,idim,ith0,jth0,i,it,k,seedv(4)
-      integer istp,nk_out,nbuf
+      integer istp,nk_out,nbuf,every,idelta,j,jstart,jnext,rtau
+      integer rskip,rdelta
       integer*8 nrate,t0,t1
       parameter(nbuf=64)
-      real*8 p,del,xb(2,mnp,nbuf)
+      real*8 p,del,xb(2,mnp,nbuf),wnew,retrial,rmage,rmage_r,pb(64)
       character*256 cfgfile,outfile
       mytid=0
       nproc=1
@@ -38,6 +48,8 @@ c move (reptation.f:4617) + metropolis_test (reptation.f:2622).
       read(11,*) del
       read(11,'(a)') cfgfile
       read(11,'(a)') outfile
+      if(mode.eq.4)read(11,*) rtau,rskip,rdelta,retrial,rmage,rmage_r
+     &                       ,every
       close(11)
       if(ntheta.ne.0)call twistg(ith0,jth0)
 c the g-vector list in use (twistg or input's untwisted list)
@@ -104,6 +116,55 @@ c the test, proposed -log psi and local energy, kept -log psi, energy
        write(6,'(a,i10,a,es16.8)') 'REFEVAL_TIMING evals ',ncfg
      &      ,' seconds ',dble(t1-t0)/dble(nrate)
        write(13) dble(t1-t0)/dble(nrate),p_new(jetot)
+      elseif(mode.eq.4)then
+       delta=del
+       ntau=rtau
+       ntauskip=rskip
+       mdelta=rdelta
+       etrial=retrial
+       mage=rmage
+       mage_r=rmage_r
+       call sigma
+       call read_path
+       write(13) n_props_in_stack,n_buffer,n_props
+       do istp=1,ncfg
+        if(mod(istp-1,every).eq.0)then
+c reptile state before the move (all n_buffer ring slots)
+         call savern(seedv)
+         write(13) seedv,jfirst,jlast
+         write(13) age,grad2_ph
+         write(13) (((x_stack(idim,ip,j),idim=1,2),ip=1,nptot)
+     &             ,j=1,n_buffer)
+         write(13) (((g_stack(idim,ip,j,iinc),idim=1,2),ip=1,nptot)
+     &             ,j=1,n_buffer)
+         write(13) ((p_stack(ip,j,iinc),ip=1,n_props_in_stack)
+     &             ,j=1,n_buffer)
+         write(13) (age_stack(j),j=1,n_buffer)
+         write(13) ((w_stack(j,k),j=1,n_buffer),k=1,5)
+         write(13) (s_stack(j),j=1,n_buffer)
+         write(13) (p_p_old(i),i=1,n_props)
+        endif
+        call reptation_move(wnew)
+c direction, beads attempted, path weight, Langevin p of each bead
+        if(idir.eq.1)then
+         jstart=jlast
+        else
+         jstart=jfirst
+        endif
+        do idelta=1,mdelta
+         pb(idelta)=-1.d0
+         if(idelta.le.ndelta)then
+          jnext=mod(jstart-1+idir*idelta+n_buffer,n_buffer)+1
+          pb(idelta)=p_stack(jacc,jnext,iinc)
+         endif
+        enddo
+        write(13) idir,ndelta
+        write(13) wnew,(pb(idelta),idelta=1,mdelta)
+        call reptation_metropolis_test(wnew)
+        write(13) age,p_p_old(jetot),p_p_old(jltf),p_p_old(jkin(0))
+       enddo
+       call savern(seedv)
+       write(13) seedv,jfirst,jlast
       endif
       close(12)
       close(13)

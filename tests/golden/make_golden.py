@@ -12,8 +12,11 @@ live under tests/golden/decks/:
   g58      generated N=58 (29+29), rs=5, u2+u3+backflow, 60 k-shells
   g122     generated N=122 (61+61), rs=10, u2+u3+backflow, 200 k-shells
   g61p     generated polarized N=61 (+ an empty second type), rs=10
+  n26r     n26 at the Gamma point without the rhok properties (reptation
+           fixtures: 14 property slots per bead)
 
-Usage: python tests/golden/make_golden.py   (needs oracle/_ref; minutes)
+Usage: python tests/golden/make_golden.py [rmc]   (needs oracle/_ref; minutes;
+       "rmc" writes only the reptation fixtures)
 """
 import importlib
 import os
@@ -36,7 +39,7 @@ WORK = "/tmp/degas_golden"
 
 def derive_n26_variants():
     src = os.path.join(DECKS, "n26")
-    for name in ("n26g", "n26p", "free26"):
+    for name in ("n26g", "n26p", "free26", "n26r"):
         dst = os.path.join(DECKS, name)
         shutil.rmtree(dst, ignore_errors=True)
         os.makedirs(dst)
@@ -51,6 +54,8 @@ def derive_n26_variants():
             if not w:
                 continue
             if w[0] == "ntwists":
+                continue
+            if name == "n26r" and w[0] == "rhok":
                 continue
             if name == "free26" and w[0] in ("u2", "u3"):
                 continue
@@ -97,6 +102,47 @@ def replay_fixture(name, nsteps, every, delta, x0):
     print("wrote", out, "accept rate", np.mean(r["age"] == 0))
 
 
+def rmc_fixture(name, nmoves, every, delta, rmc, beads):
+    """Reference reptation run (refeval mode 4): the full reptile state every
+    `every` moves and every move's draws, weights and decisions."""
+    d = deck.read_deck(os.path.join(DECKS, name))
+    prm = dict(rmc, every=every)
+    raw, _ = po.run_refeval(os.path.join(DECKS, name), os.path.join(WORK, name + "_rmc"), 4,
+                            beads, delta=delta, nsteps=nmoves, rmc=prm, deck=d)
+    r = po.parse_rmc(raw, d.nptot, nmoves, every, int(rmc["mdelta"]))
+    out = os.path.join(HERE, f"rmc_{name}.npz")
+    # the bead signs are all 1 on this path (s_new = 1); they are not stored
+    keep = {k: v for k, v in r.items() if k != "ck_s"}
+    np.savez_compressed(out, delta=delta, every=every, beads=beads,
+                        **{k: np.asarray(v) for k, v in rmc.items()}, **keep)
+    print("wrote", out, "reptation accept rate", np.mean(r["age"] == 0),
+          "bead accept", np.mean(r["pbead"][r["pbead"] >= 0]))
+
+
+def random_path(x0, ntau, sigma, seed):
+    """ntau beads: x0 followed by a Gaussian random walk of step sigma."""
+    rng = np.random.default_rng(seed)
+    steps = rng.normal(0.0, sigma, size=(ntau - 1,) + x0.shape)
+    return np.concatenate([x0[None], x0[None] + np.cumsum(steps, axis=0)])
+
+
+def make_rmc():
+    derive_n26_variants()
+    shipped = deck.read_configs([os.path.join(DECKS, "n26", f) for f in
+                                 ("n26.up.x.res.0", "n26.down.x.res.0")], [13, 13])
+    rmc_fixture("n26r", 3000, 100, 0.05,
+                dict(ntau=16, ntauskip=3, mdelta=4, etrial=-0.45, mage=1e9, mage_r=1e9),
+                random_path(shipped[0], 16, 0.02, 31))
+    L58 = deck.read_deck(os.path.join(DECKS, "g58")).el[0]
+    g = (np.arange(8) + 0.5) / 8 * L58 - L58 / 2
+    sites = np.array([(a, b) for a in g for b in g])
+    rng = np.random.default_rng(33)
+    x58 = sites[rng.permutation(64)[:58]] + rng.normal(0, 0.05 * L58 / 8, size=(58, 2))
+    rmc_fixture("g58", 500, 100, 1.0,
+                dict(ntau=10, ntauskip=2, mdelta=3, etrial=-0.3, mage=4, mage_r=6),
+                random_path(x58, 10, 0.1, 34))
+
+
 def main():
     if not po.have_ref():
         sys.exit("oracle/_ref missing: run oracle/build_ref.sh first")
@@ -138,4 +184,9 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["rmc"]:
+        os.makedirs(WORK, exist_ok=True)
+        make_rmc()
+    else:
+        main()
+        make_rmc()
