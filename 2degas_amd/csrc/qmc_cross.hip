@@ -43,8 +43,13 @@ __device__ __forceinline__ void xc_pin(xc_d4& a) { __asm__ volatile("" : "+v"(a)
 // and register v, element (row l/16 + 4v, column l%16) of the 16 x 16 tile.
 template <int G, int H>
 __device__ __forceinline__ double xc_contract(const xc_d4 (&acc)[XC_TPW], const zd* __restrict__ Fg,
-                                              int m, int lane, int nT) {
+                                              int m, int lane_in, int nT) {
   const int R = 2 * m;
+  // the addresses of F are formed here, not hoisted out of the chunk loops
+  // (where they would have to be spilled)
+  const zd* Fq = Fg;
+  int lane = lane_in;
+  __asm__ volatile("" : "+s"(Fq), "+v"(lane));
   double t = 0.0;
 #pragma unroll
   for (int q = 0; q < xc_nt(H); q++) {
@@ -58,8 +63,8 @@ __device__ __forceinline__ double xc_contract(const xc_d4 (&acc)[XC_TPW], const 
         const int r = 16 * TJ + (lane >> 4) + 4 * v;
         if (r < R && sc < R) {
           const int jr = r >> 1, ar = r & 1;
-          const zd f1 = Fg[(ar * m + ks) * m + jr];
-          const zd f2 = Fg[(bs * m + jr) * m + ks];
+          const zd f1 = Fq[(ar * m + ks) * m + jr];
+          const zd f2 = Fq[(bs * m + jr) * m + ks];
           tq = fma(f1.x * f2.x - f1.y * f2.y, acc[q][v], tq);
         }
       }
@@ -93,7 +98,9 @@ __device__ __forceinline__ void xc_body(const DevSys& s, double* __restrict__ ds
       // V chunk: 64 particle rows j (zero beyond m) x 16 particles (zero
       // beyond cn); the 2 x 2 block of (j, ip) is symmetric
       for (int e = tid; e < 64 * (XC_KC / 2); e += XC_THREADS) {
-        const int j = e >> 4, pc = e & 15;
+        // consecutive threads take consecutive rows: the column-major writes
+        // then spread over the LDS banks
+        const int j = e & 63, pc = e >> 6;
         double v11 = 0, v12 = 0, v22 = 0;
         const int ip = c0 + pc;
         if (j < m && pc < cn) {

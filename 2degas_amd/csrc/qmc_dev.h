@@ -13,7 +13,7 @@
 // k_cross: threads, V columns staged per chunk, LDS row stride (doubles)
 #define XC_THREADS 512
 #define XC_KC 32
-#define XC_RS 136
+#define XC_RS 144  // 128 rows + 16: the four 16-lane groups of an MFMA operand read disjoint banks
 
 // Type pair index of particles of types ti, tj (0: 1-1, 1: 1-2, 2: 2-2).
 __host__ __device__ inline int tpair(int ti, int tj) { return ti + tj; }

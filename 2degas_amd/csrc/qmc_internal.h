@@ -56,7 +56,7 @@ struct qmc_ctx {
   size_t lds_fgrad = 0;
   // per-kernel event timing
   bool timing = false;
-  std::vector<hipEvent_t> ev;  // 5 events per recorded launch
+  std::vector<hipEvent_t> ev;  // TIMING_EVENTS events per recorded launch
   int ev_used = 0;
   // VMC scratch: proposal buffers for vmc_nw walkers
   int vmc_nw = 0;

@@ -121,7 +121,7 @@ module qmc_hip_mod
      integer(c_int) function qmc_timing_read(ctx, ms, nlaunch) bind(C, name='qmc_timing_read')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx
-       real(c_double), intent(out) :: ms(4)
+       real(c_double), intent(out) :: ms(5)
        integer(c_int), intent(out) :: nlaunch
      end function qmc_timing_read
 

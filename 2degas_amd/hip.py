@@ -147,8 +147,9 @@ class Context:
             raise QmcError(rc, "qmc_timing")
 
     def timing_read(self):
-        """Mean ms per launch of (pairs, det, cross, final) and the launch count."""
-        ms = np.zeros(4)
+        """Mean ms per launch of (pairs, determinant, drift terms, cross,
+        final) and the launch count."""
+        ms = np.zeros(5)
         n = C.c_int(0)
         rc = lib().qmc_timing_read(self.h, abi.dptr(ms), C.byref(n))
         if rc:

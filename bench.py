@@ -24,13 +24,35 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6
 
 
+KERNELS = ["pairs", "det", "fgrad", "cross", "final"]
+
+
 def kernel_flops(nup, ndown, nk_ewald):
-    """Algorithmic flops per walker of each kernel (pairs, det, cross, final)."""
+    """Algorithmic flops per walker of each kernel (pairs, determinant,
+    drift/diagonal terms, cross, final); their sum is qmc_flops_per_eval.
+    The determinant is k_gj above 32 particles per spin (k_fgrad then does the
+    diagonal terms) and k_det, which does both, below."""
     n = nup + ndown
     pairs = 160.0 * n * (n - 1) / 2 + 5.0 * n * nk_ewald + 20.0 * nk_ewald
-    det = sum(24.0 * m ** 3 + 37.0 * m * m for m in (nup, ndown))
+    gj = max(nup, ndown) > 32
+    det = sum(24.0 * m ** 3 + (0.0 if gj else 37.0 * m * m) for m in (nup, ndown))
+    fgrad = sum(37.0 * m * m for m in (nup, ndown)) if gj else 0.0
     cross = sum(24.0 * n * m * m for m in (nup, ndown))
-    return [pairs, det, cross, 0.0]
+    return [pairs, det, fgrad, cross, 0.0]
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE summary (profiles/traffic_fp64_n122_w4096.md, copy-calibrated),
+    or None when that summary has no row for it."""
+    path = os.path.join(ROOT, "profiles", "traffic_fp64_n122_w4096.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == kernel:
+            return float(cells[5]) * 1e6
+    return None
 
 
 def cpu_baseline(deckdir, x, seconds_target=12.0):
@@ -87,6 +109,12 @@ def main():
     ap.add_argument("--shells", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--rqmc", action="store_true",
+                    help="reptation workload instead: N=58 (29+29) rs=5 backflow, 512 reptiles "
+                         "x 200 beads, delta 0.01 Ry^-1 (tau 0.02 Ha^-1); --steps moves are timed")
+    ap.add_argument("--reptiles", type=int, default=512)
+    ap.add_argument("--ntau", type=int, default=200)
+    ap.add_argument("--mdelta", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 1 GiB device-to-device copies: the known-size "
                          "kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
@@ -108,6 +136,8 @@ def main():
     hip = importlib.import_module("2degas_amd.hip")
     deck = importlib.import_module("2degas_amd.deck")
     dg = importlib.import_module("2degas_amd.deckgen")
+    if args.rqmc:
+        return bench_rqmc(args, hip, deck, dg)
     tmp = tempfile.mkdtemp(prefix="degas_bench_")
     deckdir = os.path.join(tmp, "deck")
     info = dg.make_deck(deckdir, "bench", args.nup, args.ndown, args.rs, args.shells)
@@ -140,7 +170,10 @@ def main():
     value = evals / dt
     guard = int((st.cpu().numpy() & 1).sum())
     kf = kernel_flops(args.nup, args.ndown, info["nk_ewald"])
-    names = ["pairs", "det", "cross", "final"]
+    names = KERNELS
+    gj = max(args.nup, args.ndown) > 32
+    kname = {"pairs": "k_pairs", "det": "k_gj" if gj else "k_det", "fgrad": "k_fgrad",
+             "cross": "k_cross", "final": "k_final"}
     dom = int(np.argmax(ms_k))
     achieved = kf[dom] * args.nw / (ms_k[dom] * 1e-3) / 1e12
     total_flops = hip.flops_per_eval(args.nup, args.ndown, info["nk_ewald"])
@@ -161,9 +194,9 @@ def main():
                                f"u2+u3, Ewald {args.shells} k-shells ({info['nk_ewald']} k), "
                                f"{args.nw} walkers per launch, FP64 path",
                    "walkers": args.nw, "N": N, "nk_ewald": info["nk_ewald"], "precision": "fp64"},
-        "roofline": {"bound": "mfma", "kernel": f"k_{names[dom]}", "achieved": achieved,
+        "roofline": {"bound": "mfma", "kernel": kname[names[dom]], "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None},
+                     "traffic": pmc_traffic(kname[names[dom]])},
         "kernel_ms": dict(zip(names, [float(v) for v in ms_k])),
         "eval_tflops": total_flops * value / 1e12,
         "eval_frac_fp64_peak": total_flops * value / 1e12 / FP64_PEAK_TFLOPS,
@@ -172,6 +205,67 @@ def main():
     if not args.no_cpu:
         cb = cpu_baseline(deckdir, x, args.cpu_seconds)
         out["cpu_baseline"] = cb
+    ctx.close()
+    print(json.dumps(out))
+
+
+def bench_rqmc(args, hip, deck, dg):
+    """Reptation moves/s on the RQMC workload: one move = direction and
+    ndelta drawn, mdelta batched bead evaluations of every reptile, path
+    weight and reptation test (reptation_move + reptation_metropolis_test)."""
+    import numpy as np
+    import torch
+    abi = importlib.import_module("2degas_amd.abi")
+    tmp = tempfile.mkdtemp(prefix="degas_rqmc_")
+    deckdir = os.path.join(tmp, "deck")
+    info = dg.make_deck(deckdir, "rqmc", 29, 29, 5.0, 60)
+    d = deck.read_deck(deckdir)
+    N = d.nptot
+    nr, ntau, md = args.reptiles, args.ntau, args.mdelta
+    ctx = hip.Context(d, twist=(1, 1), device=0)
+    # initial paths: a jittered lattice per reptile followed by small
+    # Gaussian steps along the imaginary-time axis
+    x0 = dg.lattice_configs(nr, N, d.el[0], 777)
+    rng = np.random.default_rng(778)
+    beads = x0[:, None] + np.cumsum(rng.normal(0.0, 0.02, size=(nr, ntau, N, 2)), axis=1)
+    beads[:, 0] = x0
+    # reptile k draws the stream of MPI rank k (seed(4) = 2*(seed(4)+k)+1 of
+    # the shipped seed 3535 3467 3262 1549, reptation.f:37-39)
+    states = [abi.limbs_to_state([3535, 3467, 3262, (2 * (1549 + k) + 1) % 4096]) for k in range(nr)]
+    prm = hip.rmc_params(0.01, -0.3, 1e30, 1e30, ntau // 10, md)
+    rep = hip.Reptiles(ctx, nr, ntau, md, states)
+    t0 = time.perf_counter()
+    rep.start(beads, prm)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter() - t0
+    rep.sweep(args.warmup, prm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep.sweep(args.steps, prm)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = rep.status()
+    moves = nr * args.steps
+    out = {
+        "metric": "reptation moves/s (RQMC, N=58)",
+        "value": moves / dt,
+        "unit": "moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"RQMC N={N} (29+29) rs=5 u2+u3+backflow, {nr} reptiles x {ntau} beads, "
+                               f"mdelta {md}, delta 0.01 (tau 0.02 Ha^-1)",
+                   "reptiles": nr, "ntau": ntau, "mdelta": md},
+        "bead_evals_per_s": nr * md * args.steps / dt,
+        "start_s": t_start,
+        "status_nonzero": int(np.sum(st != 0)),
+    }
     ctx.close()
     print(json.dumps(out))
 
