@@ -39,6 +39,17 @@ module qmc_hip_mod
      type(c_ptr) :: x, g, h, props, grad2_ph, age, rng, blk
   end type qmc_walkers
 
+  type, bind(C) :: qmc_reptiles
+     integer(c_int32_t) :: nr, ntau, nbuf
+     type(c_ptr) :: x, g, props, age, w, ends, rage, pp_old, grad2_ph, rng, blk
+  end type qmc_reptiles
+
+  type, bind(C) :: qmc_rmc_params
+     real(c_double) :: delta, etrial, mage, mage_r
+     integer(c_int32_t) :: ntauskip, mdelta, rejection, wpiu, gstorto, nodalaction, ecut, bm_order
+     real(c_double) :: value_ecut
+  end type qmc_rmc_params
+
   interface
      integer(c_int) function qmc_ctx_create(tables, device, ctx) bind(C, name='qmc_ctx_create')
        import :: c_int, c_ptr, qmc_tables
@@ -111,6 +122,22 @@ module qmc_hip_mod
        integer(c_int), value :: nw
        integer(c_int32_t), intent(out) :: sticky(*)
      end function qmc_vmc_status
+
+     integer(c_int) function qmc_rmc_start(ctx, r, prm, stream) bind(C, name='qmc_rmc_start')
+       import :: c_int, c_ptr, qmc_reptiles, qmc_rmc_params
+       type(c_ptr), value :: ctx, stream
+       type(qmc_reptiles), intent(in) :: r
+       type(qmc_rmc_params), intent(in) :: prm
+     end function qmc_rmc_start
+
+     integer(c_int) function qmc_rmc_sweep(ctx, r, prm, nsteps, trace, stream) &
+          bind(C, name='qmc_rmc_sweep')
+       import :: c_int, c_ptr, qmc_reptiles, qmc_rmc_params
+       type(c_ptr), value :: ctx, trace, stream
+       type(qmc_reptiles), intent(in) :: r
+       type(qmc_rmc_params), intent(in) :: prm
+       integer(c_int), value :: nsteps
+     end function qmc_rmc_sweep
 
      integer(c_int) function qmc_timing(ctx, enable) bind(C, name='qmc_timing')
        import :: c_int, c_ptr

@@ -141,7 +141,9 @@ int qmc_eval_batch(qmc_ctx* ctx, int nw, const double* x, double* props,
 /*
  * Same as qmc_eval_batch with all arrays already in device memory and the
  * work queued on `stream` (a hipStream_t, NULL = default stream); returns
- * once queued.  `precision` selects QMC_PREC_FP64 or QMC_PREC_MIXED.
+ * once queued.  `precision` must be QMC_PREC_FP64; QMC_PREC_MIXED is reserved
+ * and returns QMC_ERR_ARG (no FP32/FP64 split of this path keeps the local
+ * energy within 1e-6 of FP64 at a useful speedup).
  */
 int qmc_eval_batch_device(qmc_ctx* ctx, int nw, const double* d_x,
                           double* d_props, double* d_g, double* d_h,

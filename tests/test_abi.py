@@ -73,3 +73,38 @@ def test_deck_pack_layout():
     # row ngrid+1.. of the v table hold vk, then vk0 (reptation.f:4716-4724)
     t = d.tables[3]
     assert np.array_equal(ut[3, 0, t.ngrid + 1:t.ngrid + 1 + len(t.vk)], t.vk)
+
+
+def test_fortran_module_binds_every_entry_point():
+    """qmc_hip_mod.f90 carries a bind(C) interface for each declaration of
+    the C header."""
+    import re
+    src = open(os.path.join(ROOT, "2degas_amd", "fortran", "qmc_hip_mod.f90")).read()
+    bound = set(re.findall(r"bind\(C,\s*name='(\w+)'\)", src))
+    assert set(header_symbols()) <= bound
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """Sizes and field offsets of the ctypes mirrors equal the C layout
+    (a probe compiled with gcc against include/qmc_hip.h)."""
+    import ctypes
+    import subprocess
+    abi = pkg("abi")
+    structs = {"qmc_tables": abi.QmcTables, "qmc_info": abi.QmcInfo,
+               "qmc_walkers": abi.QmcWalkers, "qmc_reptiles": abi.QmcReptiles,
+               "qmc_rmc_params": abi.QmcRmcParams}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "qmc_hip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    c = tmp_path / "probe.c"
+    c.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)])
+    got = dict(l.split() for l in subprocess.check_output([str(exe)]).decode().splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, f"{cname}.{fname}"
