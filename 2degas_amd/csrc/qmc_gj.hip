@@ -13,9 +13,11 @@
 // rounding only).  Rows are never swapped and never scaled during the sweep:
 // every other row takes v -= (f / piv) u with u the raw pivot row, and the
 // pivot row is divided by its pivot once at the end, so row p of step k ends
-// up holding row k of X.  The wavefront that owns column k+1 finishes its
-// update and publishes the next pivot (column values, row, 1/piv) before the
-// step's single barrier, so the search never sits between two barriers.
+// up holding row k of X.  Each wavefront takes the pivot row's values for
+// its own columns from the pivot lane with v_readlane.  The wavefront that
+// owns column k+1 finishes its update and publishes the next pivot (column
+// values, row, 1/piv) before the step's single barrier, so the search never
+// sits between two barriers.
 //
 // k_fgrad: drift and the diagonal Laplacian terms (reptation.f:10195-10240)
 // from F: d_j = F_b(j,j) and diag(A^-1 d_a d_b A)_j = sum_k F_a(j,k) F_b(k,j)
@@ -57,8 +59,6 @@ __device__ __forceinline__ void gj_pin(zd& a) { __asm__ volatile("" : "+v"(a.x),
 struct __attribute__((aligned(16))) GjShared {
   // pivot column of the step, double-buffered
   __attribute__((aligned(16))) zd mbuf[2][GJ_MP];
-  // raw pivot row, each wavefront's own slots
-  __attribute__((aligned(16))) zd prow[GJ_WAVES][GJ_CW];
   zd pinv[GJ_MP];           // 1/pivot of step k
   double pmod[GJ_MP];       // |pivot_k|^2
   int p[2];
@@ -138,9 +138,6 @@ __global__ __launch_bounds__(GJ_THREADS, 4) void k_gj(DevSys s, WsLayout L, doub
 
   bool used = i >= m;
   int myrank = -1;
-  // per-lane address of this wavefront's pivot-row buffer: LDS reads and
-  // writes then take the slot as an immediate offset
-  zd* const prw = &sh.prow[0][0] + (tid >> 6) * GJ_CW;
   if (wave == 0) gj_publish(sh, v[0], !used, lane, 0, 0);
   __syncthreads();
 
@@ -161,31 +158,15 @@ __global__ __launch_bounds__(GJ_THREADS, 4) void k_gj(DevSys s, WsLayout L, doub
         used = true;
         myrank = k;
         nf = zmk(0.0, 0.0);
-#pragma unroll
-        for (int c = s8; c < GJ_CW; c++) prw[c] = v[c];
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      // two pivot-row values in use and the next two in flight
-      zd ua[2], ub[2];
+      // the pivot row comes straight from lane p's registers into scalar
+      // registers (v_readlane) and feeds the FMAs as a scalar operand; a store
+      // of it to LDS by that one lane costs the LDS a full wavefront per value
 #pragma unroll
-      for (int q = 0; q < 2; q++)
-        if (s8 + q < GJ_CW) ua[q] = prw[s8 + q];
-#pragma unroll
-      for (int c0 = s8; c0 < GJ_CW; c0 += 2) {
-#pragma unroll
-        for (int q = 0; q < 2; q++)
-          if (c0 + 2 + q < GJ_CW) ub[q] = prw[c0 + 2 + q];
-        __asm__ volatile("" ::: "memory");
-#pragma unroll
-        for (int q = 0; q < 2; q++)
-          if (c0 + q < GJ_CW) {
-            v[c0 + q] = zfma(nf, ua[q], v[c0 + q]);
-            gj_pin(v[c0 + q]);
-          }
-#pragma unroll
-        for (int q = 0; q < 2; q++) ua[q] = ub[q];
+      for (int c = s8; c < GJ_CW; c++) {
+        const zd u = zmk(gj_readlane(v[c].x, p), gj_readlane(v[c].y, p));
+        v[c] = zfma(nf, u, v[c]);
+        gj_pin(v[c]);
       }
       // the owner of column k+1 publishes the next pivot
       const int kn = k + 1;
