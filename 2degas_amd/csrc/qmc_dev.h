@@ -178,6 +178,21 @@ __device__ __forceinline__ double spline0(const DevSys& s, TabRef t, int e, int 
   return e < 0 ? f / p : f * p;
 }
 
+// r = sqrt(r2) and 1/r for r2 > 0: the hardware reciprocal square root
+// (about 23 correct bits) refined by two Newton steps y <- y + y (1/2 - r2 y^2 / 2),
+// which leave y within a few ulp of 1/sqrt(r2); r = r2 y.  Some 10
+// instructions where the correctly rounded sqrt and divide take over 30.
+__device__ __forceinline__ void qrsqrt(double r2, double& r, double& byr) {
+  double y = __builtin_amdgcn_rsq(r2);
+  const double h = 0.5 * r2;
+  double e = fma(-(h * y), y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-(h * y), y, 0.5);
+  y = fma(y, e, y);
+  byr = y;
+  r = r2 * y;
+}
+
 // Minimum-image pair geometry (distances, reptation.f:4138-4151):
 // u = x_i - x_j folded with nint, r, grid index and remainder, 1/r.
 struct PairGeo {
@@ -192,14 +207,30 @@ __device__ __forceinline__ PairGeo pair_geo(const DevSys& s, double xi, double y
   dy = dy - s.el[1] * round(s.eli[1] * dy);
   p.ux = dx;
   p.uy = dy;
-  p.r = sqrt(dx * dx + dy * dy);
+  qrsqrt(dx * dx + dy * dy, p.r, p.byr);
   p.ind = (int)(p.r * s.drti);
   p.rem = p.r - p.ind * s.drt;
-  p.byr = 1.0 / p.r;
   return p;
 }
 
 __device__ __forceinline__ int ptype(const DevSys& s, int i) { return i >= s.ipf[1] && s.ntypes > 1 ? 1 : 0; }
+
+// Pair loops: particle i is handled by tpp threads that split its partners j.  The
+// particles of each type occupy whole wavefronts (each type's count is padded
+// to a multiple of 64 / tpp slots), and the partners are visited type by type,
+// so the type pair of every pair loop is wavefront-uniform: the table rows,
+// grid sizes and potential exponents are scalar operands, and a type pair
+// without a table of some kind skips that term with a scalar branch.
+struct PairSlots {
+  int gsz, ns0, nslot;
+};
+__device__ __forceinline__ PairSlots pair_slots(const DevSys& s, int tpp_log2) {
+  PairSlots p;
+  p.gsz = 64 >> tpp_log2;
+  p.ns0 = (s.np[0] + p.gsz - 1) / p.gsz * p.gsz;
+  p.nslot = p.ns0 + (s.ntypes > 1 ? (s.np[1] + p.gsz - 1) / p.gsz * p.gsz : 0);
+  return p;
+}
 
 // Block-wide sum of a double over QMC_BLOCK threads; red must hold
 // QMC_BLOCK/64 doubles.  All threads get the result.

@@ -265,43 +265,54 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
   double acc_g2 = 0.0, acc_h0 = 0.0, acc_h1 = 0.0;
   double* gdet = dsc + DS_N;
   double* Qd = gdet + 2 * N;
-  for (int i0 = 0; i0 < N; i0 += ngrp) {
-    const int ip = i0 + grp;
-    const bool act = ip < N;
-    const int ti = act ? ptype(s, ip) : 0;
-    const bool on = act && (sp ? pick2(s.iub_flag[1], ti) : pick2(s.iub_flag[0], ti));
-    const bool insp = ip >= p0 && ip < p0 + m;
+  // particles in type-uniform wavefronts, partners visited type by type (as
+  // in k_pairs): the table row and whether i or c belongs to this spin are
+  // wavefront-uniform
+  const PairSlots ps = pair_slots(s, tpp_log2);
+  const int wgrp0 = __builtin_amdgcn_readfirstlane(tid >> 6) * ps.gsz;
+  for (int sl0 = 0; sl0 < ps.nslot; sl0 += ngrp) {
+    if (sl0 + wgrp0 >= ps.nslot) break;
+    const int ti = sl0 + wgrp0 >= ps.ns0 ? 1 : 0;
+    const int li = sl0 + grp - (ti ? ps.ns0 : 0);
+    const bool act = li < pick2(s.np, ti);
+    const int ip = pick2(s.ipf, ti) + li;
+    const bool on = sp ? pick2(s.iub_flag[1], ti) : pick2(s.iub_flag[0], ti);
+    const bool insp = ti == sp;
     double zxr = 0, zxi = 0, zyr = 0, zyi = 0, hq = 0, hd = 0;
     double q11 = 0, q12 = 0, q22 = 0, bx = 0, by = 0;
-    if (on) {
+    if (act && on) {
       const double xi = xs[2 * ip], yi = xs[2 * ip + 1];
-      for (int c = lane; c < N; c += tpp) {
-        if (c == ip) continue;
-        const TabRef tb = tabref<TK_B>(s, tpair(ti, ptype(s, c)));
+      for (int tj = 0; tj < s.ntypes; tj++) {
+        const TabRef tb = tabref<TK_B>(s, tpair(ti, tj));
         if (tb.row0 < 0) continue;
-        PairGeo p = pair_geo(s, xi, yi, xs[2 * c], xs[2 * c + 1]);
-        double eta, deta, ddeta;
-        spline3(s, tb, p.ind, p.rem, eta, deta, ddeta);
-        const double cb = deta * p.byr;
-        const double mb11 = eta + cb * p.ux * p.ux, mb12 = cb * p.ux * p.uy,
-                     mb22 = eta + cb * p.uy * p.uy;
-        const double cc = ddeta + 3.0 * cb;
-        if (insp) {
-          q11 += mb11; q12 += mb12; q22 += mb22;
-          bx += cc * p.ux; by += cc * p.uy;
-        }
-        if (c >= p0 && c < p0 + m) {
-          const int jf = c - p0;
-          const double dxr = dl[4 * jf], dxi = dl[4 * jf + 1], dyr = dl[4 * jf + 2],
-                       dyi = dl[4 * jf + 3];
-          zxr -= dxr * mb11 + dyr * mb12;
-          zxi -= dxi * mb11 + dyi * mb12;
-          zyr -= dxr * mb12 + dyr * mb22;
-          zyi -= dxi * mb12 + dyi * mb22;
-          hq -= cc * (dxr * p.ux + dyr * p.uy);
-          const double a11 = mb11 * mb11 + mb12 * mb12, a12 = mb12 * (mb11 + mb22),
-                       a22 = mb12 * mb12 + mb22 * mb22;
-          hd += dd[3 * jf] * a11 + 2.0 * dd[3 * jf + 1] * a12 + dd[3 * jf + 2] * a22;
+        const bool jin = tj == sp;
+        const int jb = pick2(s.ipf, tj), je = jb + pick2(s.np, tj);
+        for (int c = jb + lane; c < je; c += tpp) {
+          if (c == ip) continue;
+          PairGeo p = pair_geo(s, xi, yi, xs[2 * c], xs[2 * c + 1]);
+          double eta, deta, ddeta;
+          spline3(s, tb, p.ind, p.rem, eta, deta, ddeta);
+          const double cb = deta * p.byr;
+          const double mb11 = eta + cb * p.ux * p.ux, mb12 = cb * p.ux * p.uy,
+                       mb22 = eta + cb * p.uy * p.uy;
+          const double cc = ddeta + 3.0 * cb;
+          if (insp) {
+            q11 += mb11; q12 += mb12; q22 += mb22;
+            bx += cc * p.ux; by += cc * p.uy;
+          }
+          if (jin) {
+            const int jf = c - p0;
+            const double dxr = dl[4 * jf], dxi = dl[4 * jf + 1], dyr = dl[4 * jf + 2],
+                         dyi = dl[4 * jf + 3];
+            zxr -= dxr * mb11 + dyr * mb12;
+            zxi -= dxi * mb11 + dyi * mb12;
+            zyr -= dxr * mb12 + dyr * mb22;
+            zyi -= dxi * mb12 + dyi * mb22;
+            hq -= cc * (dxr * p.ux + dyr * p.uy);
+            const double a11 = mb11 * mb11 + mb12 * mb12, a12 = mb12 * (mb11 + mb22),
+                         a22 = mb12 * mb12 + mb22 * mb22;
+            hd += dd[3 * jf] * a11 + 2.0 * dd[3 * jf + 1] * a12 + dd[3 * jf + 2] * a22;
+          }
         }
       }
     }

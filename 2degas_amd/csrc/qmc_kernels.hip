@@ -18,13 +18,13 @@
 //             (kinetic, reptation.f:2729-2775; potential tail, 4088-4096).
 #include "qmc_dev.h"
 
-// k-vectors per thread in k_pairs (nk <= KMAXPT * QMC_BLOCK)
-#define KMAXPT 8
+// k-vectors per thread and pass in k_pairs
+#define KMAXPT 4
 
 // ---------------------------------------------------------------------------
 // k_pairs
 // LDS: xs[2N], Gs[2N], red[8], then kchunk x (nkxv + nkyv) phases
-__global__ __launch_bounds__(QMC_BLOCK) void k_pairs(DevSys s, WsLayout L,
+__global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
                                                      const double* __restrict__ x,
                                                      double* __restrict__ ws,
                                                      double* __restrict__ props, int tpp_log2) {
@@ -41,52 +41,60 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_pairs(DevSys s, WsLayout L,
   const int lane = threadIdx.x & (tpp - 1);
   const int grp = threadIdx.x >> tpp_log2;
   const int ngrp = QMC_BLOCK >> tpp_log2;
+  const PairSlots ps = pair_slots(s, tpp_log2);
+  const int wgrp0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * ps.gsz;
   double* wsw = ws + (int64_t)w * L.per_walker;
   double acc_ltf = 0.0, acc_h0 = 0.0, acc_h1 = 0.0, acc_v0 = 0.0, acc_v1 = 0.0;
 
   // pass 1: everything that needs one sweep over partners
-  for (int i0 = 0; i0 < N; i0 += ngrp) {
-    const int i = i0 + grp;
-    const bool act = i < N;
+  for (int sl0 = 0; sl0 < ps.nslot; sl0 += ngrp) {
+    if (sl0 + wgrp0 >= ps.nslot) break;
+    const int ti = sl0 + wgrp0 >= ps.ns0 ? 1 : 0;  // wavefront-uniform
+    const int li = sl0 + grp - (ti ? ps.ns0 : 0);
+    const bool act = li < pick2(s.np, ti);
+    const int i = pick2(s.ipf, ti) + li;
     double gx = 0, gy = 0, Gx = 0, Gy = 0, qx = 0, qy = 0, ltf = 0, hh = 0, vv = 0;
-    const int ti = act ? ptype(s, i) : 0;
     if (act) {
       const double xi = xs[2 * i], yi = xs[2 * i + 1];
-      for (int j = lane; j < N; j += tpp) {
-        if (j == i) continue;
-        const int tp = tpair(ti, ptype(s, j));
-        PairGeo p = pair_geo(s, xi, yi, xs[2 * j], xs[2 * j + 1]);
-        double f, df, ddf;
-        if (s.has_u2) {
-          spline3(s, tabref<TK_U2>(s, tp), p.ind, p.rem, f, df, ddf);
-          ltf += 0.5 * f;
-          const double dtr = df * p.byr;
-          gx += dtr * p.ux;
-          gy += dtr * p.uy;
-          hh += ddf + dtr;
-        }
-        if (s.has_u3) {
-          spline3(s, tabref<TK_U3>(s, tp), p.ind, p.rem, f, df, ddf);
-          Gx -= f * p.ux;
-          Gy -= f * p.uy;
-          // (xi r)^2 pair term of the three-body factor (reptation.f:2822-2832)
-          const double r = p.r;
-          const double tr = f * r;
-          const double d3 = 2.0 * f * (f + r * df);
-          const double dd3 = 2.0 * (f * f + 4.0 * r * f * df + r * r * (df * df + ddf * f));
-          ltf += 0.5 * tr * tr;
-          gx += d3 * p.ux;
-          gy += d3 * p.uy;
-          hh += dd3 + d3;
-        }
-        if (s.has_b) {
-          spline3(s, tabref<TK_B>(s, tp), p.ind, p.rem, f, df, ddf);
-          qx += f * p.ux;
-          qy += f * p.uy;
-        }
-        if (s.has_v2) {
-          vv += 0.5 * spline0(s, tabref<TK_V>(s, tp), pick3(s.vexp, tp), p.ind, p.rem, p.byr) *
-                pick3(s.v2value, tp);
+      for (int tj = 0; tj < s.ntypes; tj++) {
+        const int tp = tpair(ti, tj);
+        const TabRef tu2 = tabref<TK_U2>(s, tp), tu3 = tabref<TK_U3>(s, tp),
+                     tb = tabref<TK_B>(s, tp), tv = tabref<TK_V>(s, tp);
+        const int ve = pick3(s.vexp, tp);
+        const double vval = 0.5 * pick3(s.v2value, tp);
+        const int jb = pick2(s.ipf, tj), je = jb + pick2(s.np, tj);
+        for (int j = jb + lane; j < je; j += tpp) {
+          if (j == i) continue;
+          PairGeo p = pair_geo(s, xi, yi, xs[2 * j], xs[2 * j + 1]);
+          double f, df, ddf;
+          if (tu2.row0 >= 0) {
+            spline3(s, tu2, p.ind, p.rem, f, df, ddf);
+            ltf += 0.5 * f;
+            const double dtr = df * p.byr;
+            gx += dtr * p.ux;
+            gy += dtr * p.uy;
+            hh += ddf + dtr;
+          }
+          if (tu3.row0 >= 0) {
+            spline3(s, tu3, p.ind, p.rem, f, df, ddf);
+            Gx -= f * p.ux;
+            Gy -= f * p.uy;
+            // (xi r)^2 pair term of the three-body factor (reptation.f:2822-2832)
+            const double r = p.r;
+            const double tr = f * r;
+            const double d3 = 2.0 * f * (f + r * df);
+            const double dd3 = 2.0 * (f * f + 4.0 * r * f * df + r * r * (df * df + ddf * f));
+            ltf += 0.5 * tr * tr;
+            gx += d3 * p.ux;
+            gy += d3 * p.uy;
+            hh += dd3 + d3;
+          }
+          if (tb.row0 >= 0) {
+            spline3(s, tb, p.ind, p.rem, f, df, ddf);
+            qx += f * p.ux;
+            qy += f * p.uy;
+          }
+          if (tv.row0 >= 0) vv += vval * spline0(s, tv, ve, p.ind, p.rem, p.byr);
         }
       }
     }
@@ -120,30 +128,36 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_pairs(DevSys s, WsLayout L,
   // h_i += -|sum_k M_ik|^2 - sum_k |M_ik|^2 + sum_k a_ik u_ik.(G_i - G_k)
   // with M = xi I + (xi'/r) u u^T and a = xi'' + 3 xi'/r.
   if (s.has_u3) {
-    for (int i0 = 0; i0 < N; i0 += ngrp) {
-      const int i = i0 + grp;
-      const bool act = i < N;
+    for (int sl0 = 0; sl0 < ps.nslot; sl0 += ngrp) {
+      if (sl0 + wgrp0 >= ps.nslot) break;
+      const int ti = sl0 + wgrp0 >= ps.ns0 ? 1 : 0;
+      const int li = sl0 + grp - (ti ? ps.ns0 : 0);
+      const bool act = li < pick2(s.np, ti);
+      const int i = pick2(s.ipf, ti) + li;
       double g3x = 0, g3y = 0, e11 = 0, e12 = 0, e22 = 0, nm2 = 0, h3 = 0;
-      const int ti = act ? ptype(s, i) : 0;
       if (act) {
         const double xi = xs[2 * i], yi = xs[2 * i + 1];
         const double Gix = Gs[2 * i], Giy = Gs[2 * i + 1];
-        for (int j = lane; j < N; j += tpp) {
-          if (j == i) continue;
-          const int tp = tpair(ti, ptype(s, j));
-          PairGeo p = pair_geo(s, xi, yi, xs[2 * j], xs[2 * j + 1]);
-          double t, dt, ddt;
-          spline3(s, tabref<TK_U3>(s, tp), p.ind, p.rem, t, dt, ddt);
-          const double c = dt * p.byr;
-          const double m11 = t + c * p.ux * p.ux, m12 = c * p.ux * p.uy, m22 = t + c * p.uy * p.uy;
-          const double dGx = Gix - Gs[2 * j], dGy = Giy - Gs[2 * j + 1];
-          g3x += m11 * dGx + m12 * dGy;
-          g3y += m12 * dGx + m22 * dGy;
-          e11 += m11;
-          e12 += m12;
-          e22 += m22;
-          nm2 += m11 * m11 + 2.0 * m12 * m12 + m22 * m22;
-          h3 += (ddt + 3.0 * c) * (p.ux * dGx + p.uy * dGy);
+        for (int tj = 0; tj < s.ntypes; tj++) {
+          const TabRef tu3 = tabref<TK_U3>(s, tpair(ti, tj));
+          if (tu3.row0 < 0) continue;
+          const int jb = pick2(s.ipf, tj), je = jb + pick2(s.np, tj);
+          for (int j = jb + lane; j < je; j += tpp) {
+            if (j == i) continue;
+            PairGeo p = pair_geo(s, xi, yi, xs[2 * j], xs[2 * j + 1]);
+            double t, dt, ddt;
+            spline3(s, tu3, p.ind, p.rem, t, dt, ddt);
+            const double c = dt * p.byr;
+            const double m11 = t + c * p.ux * p.ux, m12 = c * p.ux * p.uy, m22 = t + c * p.uy * p.uy;
+            const double dGx = Gix - Gs[2 * j], dGy = Giy - Gs[2 * j + 1];
+            g3x += m11 * dGx + m12 * dGy;
+            g3y += m12 * dGx + m22 * dGy;
+            e11 += m11;
+            e12 += m12;
+            e22 += m22;
+            nm2 += m11 * m11 + 2.0 * m12 * m12 + m22 * m22;
+            h3 += (ddt + 3.0 * c) * (p.ux * dGx + p.uy * dGy);
+          }
         }
       }
       for (int o = 1; o < tpp; o <<= 1) {
@@ -169,89 +183,111 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_pairs(DevSys s, WsLayout L,
   double pk0 = 0.0, pk1 = 0.0, pkt = 0.0;
   if (s.nrhok) {
     double* pw = props + (int64_t)w * s.n_props;
-    const int kpt = (s.nk + QMC_BLOCK - 1) / QMC_BLOCK;
-    double rc0[KMAXPT], rc1[KMAXPT], rs0[KMAXPT], rs1[KMAXPT];
+    // passes of KMAXPT k-vectors per thread
+    for (int kb = 0; kb < s.nk; kb += KMAXPT * QMC_BLOCK) {
+      double rc0[KMAXPT], rc1[KMAXPT], rs0[KMAXPT], rs1[KMAXPT];
+      // k-vector of slot q of this thread (-1: none).  The factorized path
+      // visits the k-vectors in (kx, ky) component order (kxy entries hold
+      // ix | iy << 8 | k << 16, sorted by ix then iy), so the threads of a
+      // wavefront share kx phases and read consecutive ky phases from LDS.
+      int kq[KMAXPT];
 #pragma unroll
-    for (int q = 0; q < KMAXPT; q++) rc0[q] = rc1[q] = rs0[q] = rs1[q] = 0.0;
-    if (s.nkxv > 0) {
-      // e^{i k.x} = e^{i kx x} e^{i ky y}: per particle chunk, the distinct
-      // x and y phases go to LDS, then each thread multiplies its k's pairs
-      double2* ph = (double2*)(sm + 4 * N + 8);
-      const int nv = s.nkxv + s.nkyv;
-      for (int c0 = 0; c0 < N; c0 += s.kchunk) {
-        const int cn = min(s.kchunk, N - c0);
-        __syncthreads();
-        for (int e = threadIdx.x; e < cn * nv; e += QMC_BLOCK) {
-          const int pc = e / nv, v = e - pc * nv;
-          const int i = c0 + pc;
-          const double arg = v < s.nkxv ? s.kxv[v] * xs[2 * i] : s.kyv[v - s.nkxv] * xs[2 * i + 1];
-          double sv, cv;
-          sincos(arg, &sv, &cv);
-          ph[e] = make_double2(cv, sv);
+      for (int q = 0; q < KMAXPT; q++) {
+        rc0[q] = rc1[q] = rs0[q] = rs1[q] = 0.0;
+        const int n = kb + threadIdx.x + q * QMC_BLOCK;
+        kq[q] = n >= s.nk ? -1 : (s.nkxv > 0 ? (s.kxy[n] >> 16) : n);
+      }
+      if (s.nkxv > 0) {
+        // e^{i k.x} = e^{i kx x} e^{i ky y}: per particle chunk, the distinct
+        // x and y phases go to LDS, then each thread multiplies its k's pairs
+        double2* ph = (double2*)(sm + 4 * N + 8);
+        const int nv = s.nkxv + s.nkyv;
+        for (int c0 = 0; c0 < N; c0 += s.kchunk) {
+          const int cn = min(s.kchunk, N - c0);
+          __syncthreads();
+          for (int e = threadIdx.x; e < cn * nv; e += QMC_BLOCK) {
+            const int pc = e / nv, v = e - pc * nv;
+            const int i = c0 + pc;
+            const double arg = v < s.nkxv ? s.kxv[v] * xs[2 * i] : s.kyv[v - s.nkxv] * xs[2 * i + 1];
+            double sv, cv;
+            qsincos(arg, &sv, &cv);
+            ph[e] = make_double2(cv, sv);
+          }
+          __syncthreads();
+          const int split = s.ntypes > 1 ? min(max(s.ipf[1] - c0, 0), cn) : cn;
+#pragma unroll
+          for (int q = 0; q < KMAXPT; q++) {
+            if (kq[q] < 0) continue;
+            const int code = s.kxy[kb + threadIdx.x + q * QMC_BLOCK];
+            const int ix = code & 0xff, iy = ((code >> 8) & 0xff) + s.nkxv;
+            // particles of type 0 come first: [0, split) of the chunk
+            for (int pc = 0; pc < split; pc++) {
+              const double2 a = ph[pc * nv + ix], bb = ph[pc * nv + iy];
+              rc0[q] += a.x * bb.x - a.y * bb.y;
+              rs0[q] += a.x * bb.y + a.y * bb.x;
+            }
+            for (int pc = split; pc < cn; pc++) {
+              const double2 a = ph[pc * nv + ix], bb = ph[pc * nv + iy];
+              rc1[q] += a.x * bb.x - a.y * bb.y;
+              rs1[q] += a.x * bb.y + a.y * bb.x;
+            }
+          }
         }
-        __syncthreads();
+      } else {
 #pragma unroll
         for (int q = 0; q < KMAXPT; q++) {
-          const int k = threadIdx.x + q * QMC_BLOCK;
-          if (q >= kpt || k >= s.nk) continue;
-          const int code = s.kxy[k];
-          const int ix = code & 0xffff, iy = (code >> 16) + s.nkxv;
-          for (int pc = 0; pc < cn; pc++) {
-            const int t = ptype(s, c0 + pc);
-            const double2 a = ph[pc * nv + ix], bb = ph[pc * nv + iy];
-            const double cr = a.x * bb.x - a.y * bb.y, ci = a.x * bb.y + a.y * bb.x;
-            if (t == 0) { rc0[q] += cr; rs0[q] += ci; }
-            else { rc1[q] += cr; rs1[q] += ci; }
+          const int k = kq[q];
+          if (k < 0) continue;
+          const double kx = s.kvec[2 * k], ky = s.kvec[2 * k + 1];
+          const int split = s.ntypes > 1 ? s.ipf[1] : N;
+          for (int i = 0; i < split; i++) {
+            double sv, cv;
+            qsincos(kx * xs[2 * i] + ky * xs[2 * i + 1], &sv, &cv);
+            rc0[q] += cv;
+            rs0[q] += sv;
+          }
+          for (int i = split; i < N; i++) {
+            double sv, cv;
+            qsincos(kx * xs[2 * i] + ky * xs[2 * i + 1], &sv, &cv);
+            rc1[q] += cv;
+            rs1[q] += sv;
           }
         }
       }
-    } else {
 #pragma unroll
       for (int q = 0; q < KMAXPT; q++) {
-        const int k = threadIdx.x + q * QMC_BLOCK;
-        if (q >= kpt || k >= s.nk) continue;
-        const double kx = s.kvec[2 * k], ky = s.kvec[2 * k + 1];
-        for (int i = 0; i < N; i++) {
-          double sv, cv;
-          sincos(kx * xs[2 * i] + ky * xs[2 * i + 1], &sv, &cv);
-          if (ptype(s, i) == 0) { rc0[q] += cv; rs0[q] += sv; }
-          else { rc1[q] += cv; rs1[q] += sv; }
+        const int k = kq[q];
+        if (k < 0) continue;
+        if (!s.irhok[0]) rc0[q] = rs0[q] = 0.0;
+        if (s.ntypes < 2 || !s.irhok[1]) rc1[q] = rs1[q] = 0.0;
+        if (s.irhok[0]) {
+          const int slot = s.jrhok + (s.irhok[0] - 1) * 2 * s.nk + 2 * k;
+          pw[slot] = rc0[q];
+          pw[slot + 1] = rs0[q];
         }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < KMAXPT; q++) {
-      const int k = threadIdx.x + q * QMC_BLOCK;
-      if (q >= kpt || k >= s.nk) continue;
-      if (!s.irhok[0]) rc0[q] = rs0[q] = 0.0;
-      if (s.ntypes < 2 || !s.irhok[1]) rc1[q] = rs1[q] = 0.0;
-      if (s.irhok[0]) {
-        const int slot = s.jrhok + (s.irhok[0] - 1) * 2 * s.nk + 2 * k;
-        pw[slot] = rc0[q];
-        pw[slot + 1] = rs0[q];
-      }
-      if (s.ntypes > 1 && s.irhok[1]) {
-        const int slot = s.jrhok + (s.irhok[1] - 1) * 2 * s.nk + 2 * k;
-        pw[slot] = rc1[q];
-        pw[slot + 1] = rs1[q];
-      }
-      // ordered type pairs (0,0), (0,1), (1,0), (1,1) (reptation.f:4034-4052)
-      if (k < s.nke[0]) {
-        const double aux = s.vk[0][k] * (rc0[q] * rc0[q] + rs0[q] * rs0[q]);
-        pk0 += aux;
-        pkt += aux;
-      }
-      if (s.ntypes > 1) {
-        if (k < s.nke[1]) {
-          const double aux = s.vk[1][k] * (rc0[q] * rc1[q] + rs0[q] * rs1[q]);
+        if (s.ntypes > 1 && s.irhok[1]) {
+          const int slot = s.jrhok + (s.irhok[1] - 1) * 2 * s.nk + 2 * k;
+          pw[slot] = rc1[q];
+          pw[slot + 1] = rs1[q];
+        }
+        // ordered type pairs (0,0), (0,1), (1,0), (1,1) (reptation.f:4034-4052)
+        if (k < s.nke[0]) {
+          const double aux = s.vk[0][k] * (rc0[q] * rc0[q] + rs0[q] * rs0[q]);
           pk0 += aux;
-          pk1 += aux;
-          pkt += 2.0 * aux;
-        }
-        if (k < s.nke[2]) {
-          const double aux = s.vk[2][k] * (rc1[q] * rc1[q] + rs1[q] * rs1[q]);
-          pk1 += aux;
           pkt += aux;
+        }
+        if (s.ntypes > 1) {
+          if (k < s.nke[1]) {
+            const double aux = s.vk[1][k] * (rc0[q] * rc1[q] + rs0[q] * rs1[q]);
+            pk0 += aux;
+            pk1 += aux;
+            pkt += 2.0 * aux;
+          }
+          if (k < s.nke[2]) {
+            const double aux = s.vk[2][k] * (rc1[q] * rc1[q] + rs1[q] * rs1[q]);
+            pk1 += aux;
+            pkt += aux;
+          }
         }
       }
     }
