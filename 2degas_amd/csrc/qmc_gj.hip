@@ -30,28 +30,6 @@
 #define GJ_MP 64
 #define GJ_THREADS (GJ_WAVES * 64)
 
-// Maximum over the wavefront of a 32-bit key: DPP swaps inside quads, row
-// rotations by 4 and 8 inside each row of 16 lanes, then the four row maxima.
-// Every lane must be active.
-__device__ __forceinline__ unsigned gj_wave_max(unsigned x) {
-  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));
-  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));
-  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x124, 0xF, 0xF, false));
-  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x128, 0xF, 0xF, false));
-  const unsigned r0 = __builtin_amdgcn_readlane((int)x, 0);
-  const unsigned r1 = __builtin_amdgcn_readlane((int)x, 16);
-  const unsigned r2 = __builtin_amdgcn_readlane((int)x, 32);
-  const unsigned r3 = __builtin_amdgcn_readlane((int)x, 48);
-  return max(max(r0, r1), max(r2, r3));
-}
-
-__device__ __forceinline__ double gj_readlane(double v, int l) {
-  const long long b = __double_as_longlong(v);
-  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)b, l);
-  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
 // Empty statement that takes a value in registers: the arithmetic producing
 // it happens before this point and is not sunk past it.
 __device__ __forceinline__ void gj_pin(zd& a) { __asm__ volatile("" : "+v"(a.x), "+v"(a.y)); }
