@@ -30,8 +30,10 @@ KERNELS = ["pairs", "det", "fgrad", "cross", "final"]
 def kernel_flops(nup, ndown, nk_ewald):
     """Algorithmic flops per walker of each kernel (pairs, determinant,
     drift/diagonal terms, cross, final); their sum is qmc_flops_per_eval.
-    The determinant is k_gj above 32 particles per spin (k_fgrad then does the
-    diagonal terms) and k_det, which does both, below."""
+    The determinant is k_gjm (matrix cores) above 32 particles per spin
+    (k_fgrad then does the diagonal terms) and k_det, which does both, below;
+    the determinant's count is the Gauss-Jordan solve's, whichever kernel
+    runs it."""
     n = nup + ndown
     pairs = 160.0 * n * (n - 1) / 2 + 5.0 * n * nk_ewald + 20.0 * nk_ewald
     gj = max(nup, ndown) > 32
@@ -180,7 +182,8 @@ def main():
     kf = kernel_flops(args.nup, args.ndown, info["nk_ewald"])
     names = KERNELS
     gj = max(args.nup, args.ndown) > 32
-    kname = {"pairs": "k_pairs", "det": "k_gj" if gj else "k_det", "fgrad": "k_fgrad",
+    det_kernel = ("k_gj" if os.environ.get("QMC_GJ_VALU") else "k_gjm") if gj else "k_det"
+    kname = {"pairs": "k_pairs", "det": det_kernel, "fgrad": "k_fgrad",
              "cross": "k_cross", "final": "k_final"}
     dom = int(np.argmax(ms_k))
     achieved = kf[dom] * args.nw / (ms_k[dom] * 1e-3) / 1e12
