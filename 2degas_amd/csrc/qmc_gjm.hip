@@ -30,7 +30,11 @@
 // F_b(k, .) = (S G'_b)(p_k, .) / piv_k with G'_b(k', j) = i g^b_{p_k'} A(p_k', j):
 // two complex 64 x 64 x 64 products on the matrix cores, S taken from the
 // LDS copy and G'_b regenerated from q and g.  log|det A| = sum_k log|piv_k|
-// (zgedi's det, reptation.f:9412-9430).
+// (zgedi's det, reptation.f:9412-9430).  The per-orbital terms of the drift
+// and diagonal Laplacian (reptation.f:10195-10240) need F_b(j,j) and
+// diag(F_a F_b) = diag(A^-1 G_a G_b A); both are formed here and stored
+// beside the determinant scalars (7 doubles per orbital), so the next kernel
+// does not read F.
 #include "qmc_dev.h"
 
 typedef double gm_d4 __attribute__((ext_vector_type(4)));
@@ -236,15 +240,53 @@ __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, dou
     __syncthreads();
   }
 
-  // ---- phase 2: H_b = S G'_b for output column tile ct (particles j), row
-  // tiles 2 rh + t: h[t][b][0] real, h[t][b][1] imaginary
+  // ---- phase 2
+  const int j = col;
+  const double qx = sh.q[2 * j], qy = sh.q[2 * j + 1];
+  double* Fd = ws + (int64_t)w * L.per_walker + pick2(L.off_det, sp) + DS_N + 2 * s.nptot + 3 * m;
+  // diag(F_a F_b)_j = diag(A^-1 G_a G_b A)_j = sum_k' S(p_j, k') (-g^a g^b)_{p_k'} A(p_k', j) / piv_j,
+  // O(m^2) in all: the wavefronts of row half 0, lane (lr, lc) summing
+  // k' = lr mod 4 for particle j, then the four partial sums combined
+  if (rh == 0) {
+    zd zxx = zmk(0.0, 0.0), zxy = zmk(0.0, 0.0), zyy = zmk(0.0, 0.0);
+    if (j < m) {
+      const int prj = sh.prow[j], swj = gm_swz(prj);
+      for (int kp = lr; kp < m; kp += 4) {
+        const int o = sh.prow[kp];
+        const double gx = sh.g[2 * o], gy = sh.g[2 * o + 1];
+        double sv, cv;
+        qsincos(gx * qx + gy * qy, &sv, &cv);
+        const int ad = prj * 64 + (kp ^ swj);
+        const zd wv = zmul(zmk(sh.Mre[ad], sh.Mim[ad]), zmk(cv, sv));
+        const double cxx = -gx * gx, cxy = -gx * gy, cyy = -gy * gy;
+        zxx = zmk(fma(cxx, wv.x, zxx.x), fma(cxx, wv.y, zxx.y));
+        zxy = zmk(fma(cxy, wv.x, zxy.x), fma(cxy, wv.y, zxy.y));
+        zyy = zmk(fma(cyy, wv.x, zyy.x), fma(cyy, wv.y, zyy.y));
+      }
+    }
+#pragma unroll
+    for (int o = 16; o < 64; o <<= 1) {
+      zxx.x += __shfl_xor(zxx.x, o, 64);
+      zxx.y += __shfl_xor(zxx.y, o, 64);
+      zxy.x += __shfl_xor(zxy.x, o, 64);
+      zxy.y += __shfl_xor(zxy.y, o, 64);
+      zyy.x += __shfl_xor(zyy.x, o, 64);
+      zyy.y += __shfl_xor(zyy.y, o, 64);
+    }
+    if (j < m && lr == 0) {
+      const zd pi = sh.pinv[j];
+      Fd[7 * j + 4] = zmul(zxx, pi).x;
+      Fd[7 * j + 5] = zmul(zxy, pi).x;
+      Fd[7 * j + 6] = zmul(zyy, pi).x;
+    }
+  }
+  // H_b = S G'_b for output column tile ct (particles j), row tiles 2 rh + t:
+  // h[t][b][0] real, h[t][b][1] imaginary
   gm_d4 h[2][2][2];
 #pragma unroll
   for (int t = 0; t < 2; t++)
 #pragma unroll
     for (int b = 0; b < 2; b++) h[t][b][0] = h[t][b][1] = gm_d4{0.0, 0.0, 0.0, 0.0};
-  const int j = col;
-  const double qx = sh.q[2 * j], qy = sh.q[2 * j + 1];
   for (int ks = 0; 4 * ks < m; ks++) {
     const int kp = 4 * ks + lr;
     double bx_re = 0.0, bx_im = 0.0, by_re = 0.0, by_im = 0.0;
@@ -292,7 +334,15 @@ __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, dou
           const int k = sh.rank[r];
           const zd pi = sh.pinv[k];
 #pragma unroll
-          for (int b = 0; b < 2; b++) Fg[(b * m + k) * m + j] = zmul(zmk(h[t][b][0][v], h[t][b][1][v]), pi);
+          for (int b = 0; b < 2; b++) {
+            const zd f = zmul(zmk(h[t][b][0][v], h[t][b][1][v]), pi);
+            Fg[(b * m + k) * m + j] = f;
+            // the diagonal F_b(j, j) also goes to the per-orbital terms
+            if (k == j) {
+              Fd[7 * j + 2 * b] = f.x;
+              Fd[7 * j + 2 * b + 1] = f.y;
+            }
+          }
         }
       }
   }

@@ -182,7 +182,7 @@ def main():
     kf = kernel_flops(args.nup, args.ndown, info["nk_ewald"])
     names = KERNELS
     gj = max(args.nup, args.ndown) > 32
-    det_kernel = ("k_gj" if os.environ.get("QMC_GJ_VALU") else "k_gjm") if gj else "k_det"
+    det_kernel = "k_gjm" if gj else "k_det"
     kname = {"pairs": "k_pairs", "det": det_kernel, "fgrad": "k_fgrad",
              "cross": "k_cross", "final": "k_final"}
     dom = int(np.argmax(ms_k))
