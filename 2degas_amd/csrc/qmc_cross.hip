@@ -15,7 +15,8 @@
 // (8-g and g+1 tiles), split 5 + 4 between the pair.  V_t is staged in LDS 16
 // particles (32 columns) at a time, column-major so each MFMA operand is one
 // 8-byte LDS read per lane.  M is formed from F (L2) tile by tile when S_t is
-// complete, so only the S accumulators stay in registers through the chunks.
+// complete, once for both particle types, so only the S accumulators stay in
+// registers through the chunks and F is read once per (walker, spin).
 #include "qmc_dev.h"
 
 typedef double xc_d4 __attribute__((ext_vector_type(4)));
@@ -38,26 +39,27 @@ __host__ __device__ constexpr bool xc_needs(int g, int h, int b) {
 // produce it are issued before this point.
 __device__ __forceinline__ void xc_pin(xc_d4& a) { __asm__ volatile("" : "+v"(a)); }
 
-// sum over the wavefront's tiles of M o S, M formed from F here (weight 2 off
-// the diagonal).  The accumulator of v_mfma_f64_16x16x4f64 holds, in lane l
-// and register v, element (row l/16 + 4v, column l%16) of the 16 x 16 tile.
+// Sums over the wavefront's tiles of M o S_0 and M o S_1, M formed from F
+// here once for both particle types (weight 2 off the diagonal).  The
+// accumulator of v_mfma_f64_16x16x4f64 holds, in lane l and register v,
+// element (row l/16 + 4v, column l%16) of the 16 x 16 tile.
 template <int G, int H>
-__device__ __forceinline__ double xc_contract(const xc_d4 (&acc)[XC_TPW], const zd* __restrict__ Fg,
-                                              int m, int lane_in, int nT) {
+__device__ __forceinline__ void xc_contract(const xc_d4 (&acc)[2][XC_TPW], const zd* __restrict__ Fg, int m,
+                                            int lane_in, int nT, double& t0, double& t1) {
   const int R = 2 * m;
   // the addresses of F are formed here, not hoisted out of the chunk loops
   // (where they would have to be spilled)
   const zd* Fq = Fg;
   int lane = lane_in;
   __asm__ volatile("" : "+s"(Fq), "+v"(lane));
-  double t = 0.0;
+  t0 = t1 = 0.0;
 #pragma unroll
   for (int q = 0; q < xc_nt(H); q++) {
     const int TJ = xc_tj(G, xc_i0(H) + q), TK = xc_tk(G, xc_i0(H) + q);
     if (TK < nT) {
       const int sc = 16 * TK + (lane & 15);
       const int ks = sc >> 1, bs = sc & 1;
-      double tq = 0.0;
+      double u0 = 0.0, u1 = 0.0;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int r = 16 * TJ + (lane >> 4) + 4 * v;
@@ -65,15 +67,17 @@ __device__ __forceinline__ double xc_contract(const xc_d4 (&acc)[XC_TPW], const 
           const int jr = r >> 1, ar = r & 1;
           const zd f1 = Fq[(ar * m + ks) * m + jr];
           const zd f2 = Fq[(bs * m + jr) * m + ks];
-          tq = fma(f1.x * f2.x - f1.y * f2.y, acc[q][v], tq);
+          const double mv = f1.x * f2.x - f1.y * f2.y;
+          u0 = fma(mv, acc[0][q][v], u0);
+          u1 = fma(mv, acc[1][q][v], u1);
         }
       }
-      t += TJ == TK ? tq : 2.0 * tq;
+      t0 += TJ == TK ? u0 : 2.0 * u0;
+      t1 += TJ == TK ? u1 : 2.0 * u1;
     }
     // one tile's F values in flight at a time
     __asm__ volatile("" ::: "memory");
   }
-  return t;
 }
 
 // Per-wavefront body for tile group (G, H): every chunk of V_t is staged by
@@ -85,12 +89,17 @@ __device__ __forceinline__ void xc_body(const DevSys& s, double* __restrict__ ds
   const int lane = tid & 63;
   const int nT = (2 * m + 15) >> 4;
   const double* base = Vt + (lane >> 4) * XC_RS + (lane & 15);
-  double xc[2] = {0.0, 0.0};
-  for (int t = 0; t < s.ntypes; t++) {
-    if (!(sp ? pick2(s.iub_flag[1], t) : pick2(s.iub_flag[0], t)) || pick2(s.np, t) == 0) continue;
-    xc_d4 acc[XC_TPW];
+  // S_t accumulators of both particle types (zero for a type without
+  // backflow to this spin), contracted with M in one pass over F at the end
+  xc_d4 acc[2][XC_TPW];
 #pragma unroll
-    for (int q = 0; q < XC_TPW; q++) acc[q] = xc_d4{0.0, 0.0, 0.0, 0.0};
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int q = 0; q < XC_TPW; q++) acc[t][q] = xc_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    if (t >= s.ntypes) break;
+    if (!(sp ? pick2(s.iub_flag[1], t) : pick2(s.iub_flag[0], t)) || pick2(s.np, t) == 0) continue;
     const int ia = pick2(s.ipf, t), ib = ia + pick2(s.np, t);
     const TabRef tb = tabref<TK_B>(s, tpair(sp, t));
     for (int c0 = ia; c0 < ib; c0 += XC_KC / 2) {
@@ -136,20 +145,23 @@ __device__ __forceinline__ void xc_body(const DevSys& s, double* __restrict__ ds
 #pragma unroll
         for (int q = 0; q < xc_nt(H); q++) {
           const int TJ = xc_tj(G, xc_i0(H) + q), TK = xc_tk(G, xc_i0(H) + q);
-          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(blk[TJ], blk[TK], acc[q], 0, 0, 0);
+          acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(blk[TJ], blk[TK], acc[t][q], 0, 0, 0);
         }
 #pragma unroll
-        for (int q = 0; q < xc_nt(H); q++) xc_pin(acc[q]);
+        for (int q = 0; q < xc_nt(H); q++) xc_pin(acc[t][q]);
         // the next step's operands are read after this step's MFMAs issue
         __asm__ volatile("" ::: "memory");
       }
       __syncthreads();
     }
-    xc[t] = block_sum<XC_THREADS>(xc_contract<G, H>(acc, Fg, m, lane, nT), red);
   }
+  double u0, u1;
+  xc_contract<G, H>(acc, Fg, m, lane, nT, u0, u1);
+  u0 = block_sum<XC_THREADS>(u0, red);
+  u1 = block_sum<XC_THREADS>(u1, red);
   if (tid == 0) {
-    dsc[DS_X0] = xc[0];
-    dsc[DS_X1] = xc[1];
+    dsc[DS_X0] = u0;
+    dsc[DS_X1] = u1;
   }
 }
 
