@@ -60,8 +60,9 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
       const double xi = xs[2 * ip], yi = xs[2 * ip + 1];
       for (int tj = 0; tj < s.ntypes; tj++) {
         const TabRef tb = tabref<TK_B>(s, tpair(ti, tj));
-        if (tb.row0 < 0) continue;
         const bool jin = tj == sp;
+        // pairs with neither particle in this spin contribute nothing
+        if (tb.row0 < 0 || !(insp || jin)) continue;
         const int jb = pick2(s.ipf, tj), je = jb + pick2(s.np, tj);
         for (int c = jb + lane; c < je; c += tpp) {
           if (c == ip) continue;

@@ -88,9 +88,11 @@ __device__ __forceinline__ void gm_put_tiles(GmShared& sh, const gm_d4 (&acc)[2]
 
 // The pivot steps of the block of columns k0..k0+nb-1 on its panel P (lane =
 // row, every lane of the wavefront active): pivot rows into pj and the LDS
-// pivot tables, the block's transform into Fp (F'(r, j) in lane r).
-__device__ __forceinline__ void gm_pivot_steps(GmShared& sh, zd (&P)[4], zd (&Fp)[4], int (&pj)[4], bool& used,
-                                               int k0, int nb, int lane) {
+// pivot tables, the block's transform into Fp (F'(r, j) in lane r), and the
+// pivot rows as they were before the block into Ur (lane = column), loaded
+// from the LDS copy as soon as each pivot is known.
+__device__ __forceinline__ void gm_pivot_steps(GmShared& sh, zd (&P)[4], zd (&Fp)[4], zd (&Ur)[4], int (&pj)[4],
+                                               bool& used, int k0, int nb, int lane) {
 #pragma unroll
   for (int j = 0; j < 4; j++) Fp[j] = zmk(0.0, 0.0);
 #pragma unroll
@@ -101,6 +103,10 @@ __device__ __forceinline__ void gm_pivot_steps(GmShared& sh, zd (&P)[4], zd (&Fp
       const unsigned key = used ? 0u : ((((top >> 6) + 1u) << 6) | (unsigned)(63 - lane));
       const int p = 63 - (int)(gj_wave_max(key) & 63u);
       pj[j] = p;
+      {
+        const int a = p * 64 + (lane ^ gm_swz(p));
+        Ur[j] = zmk(sh.Mre[a], sh.Mim[a]);
+      }
       const zd pv = zmk(gj_readlane(P[j].x, p), gj_readlane(P[j].y, p));
       const double d = pv.x * pv.x + pv.y * pv.y;
       const double rd = gm_rcp(d);
@@ -187,15 +193,14 @@ __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, dou
       }
       bool used = sh.used[lane] != 0;
       int pj[4] = {0, 0, 0, 0};
-      gm_pivot_steps(sh, P, Fp, pj, used, k0, nb, lane);
+      zd Ur[4];
+      gm_pivot_steps(sh, P, Fp, Ur, pj, used, k0, nb, lane);
       sh.used[lane] = used ? 1 : 0;
-      // the block's pivot rows as they are before the block (lane = column)
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (j < nb) {
-          const int a = pj[j] * 64 + (lane ^ gm_swz(pj[j]));
-          sh.Ure[j * 64 + lane] = sh.Mre[a];
-          sh.Uim[j * 64 + lane] = sh.Mim[a];
+          sh.Ure[j * 64 + lane] = Ur[j].x;
+          sh.Uim[j * 64 + lane] = Ur[j].y;
         }
       }
 #pragma unroll
