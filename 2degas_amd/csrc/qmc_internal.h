@@ -51,7 +51,7 @@ struct qmc_ctx {
   hipStream_t stream = nullptr;
   int kc = 16;
   size_t lds_pairs = 0, lds_det = 0, lds_cross = 0;
-  int tpp_pairs = 0, tpp_det = 0;
+  int tpp_det = 0;
   bool use_gj = false;   // matrix-core Gauss-Jordan determinant path (33..64 per spin)
   size_t lds_gjm = 0;
   size_t lds_fgrad = 0;

@@ -23,160 +23,208 @@
 
 // ---------------------------------------------------------------------------
 // k_pairs
-// LDS: xs[2N], Gs[2N], red[8], then kchunk x (nkxv + nkyv) phases
+// LDS: xs[2N], Gs[2N], red[8], then a region shared by the per-wavefront
+// partial sums of the pair passes (4 x 6 x N doubles) and, afterwards, the
+// kchunk x (nkxv + nkyv) phases of rho_k.
+//
+// The pair passes visit every unordered pair once, on a round-robin
+// schedule: the particles of type t sit in lanes 0..np_t-1 of a block of 64
+// (np_t <= 64), and step d of block pair (I, J) pairs lane l of I with lane
+// (l + d) mod 64 of J, so within a step every lane has a distinct partner.
+// Within one type d runs over 1..32 (d = 32 on lanes 0..31 only), across the
+// two types over 0..63; the steps are split into four contiguous runs, one
+// per wavefront.  A pair's contribution to i stays in registers until the
+// wavefront's run leaves block I; its contribution to j is added to the
+// wavefront's own partial sums in LDS (distinct j within a step, so no two
+// lanes touch one address), and the four partial sums are added in a fixed
+// order afterwards: the result does not depend on scheduling.
+#define PP_NQ 6  // per-particle sums of a pass
+
+// Step u of the schedule: block (particle type) pair and lane offset.
+__device__ __forceinline__ void rr_unit(int u, bool two, int t1, int& ti, int& tj, int& d) {
+  if (!two) { ti = tj = t1; d = u + 1; }
+  else if (u < 32) { ti = tj = 0; d = u + 1; }
+  else if (u < 64) { ti = tj = 1; d = u - 31; }
+  else { ti = 0; tj = 1; d = u - 64; }
+}
+
+// part[q * N + p] += v for the wavefront's partial sums
+__device__ __forceinline__ void pp_add(double* part, int N, int p, const double (&v)[PP_NQ], int nq) {
+#pragma unroll
+  for (int q = 0; q < PP_NQ; q++)
+    if (q < nq) part[q * N + p] += v[q];
+}
+
 __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
                                                      const double* __restrict__ x,
                                                      double* __restrict__ ws,
-                                                     double* __restrict__ props, int tpp_log2) {
+                                                     double* __restrict__ props) {
   extern __shared__ double sm[];
   const int w = blockIdx.x;
   const int N = s.nptot;
   double* xs = sm;
   double* Gs = sm + 2 * N;
   double* red = sm + 4 * N;
+  double* parts = sm + 4 * N + 8;
   const double* xw = x + (int64_t)w * 2 * N;
-  for (int k = threadIdx.x; k < 2 * N; k += QMC_BLOCK) xs[k] = xw[k];
+  const int tid = threadIdx.x, l = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  double* part = parts + wv * PP_NQ * N;
+  for (int k = tid; k < 2 * N; k += QMC_BLOCK) xs[k] = xw[k];
+  for (int k = tid; k < 4 * PP_NQ * N; k += QMC_BLOCK) parts[k] = 0.0;
   __syncthreads();
-  const int tpp = 1 << tpp_log2;
-  const int lane = threadIdx.x & (tpp - 1);
-  const int grp = threadIdx.x >> tpp_log2;
-  const int ngrp = QMC_BLOCK >> tpp_log2;
-  const PairSlots ps = pair_slots(s, tpp_log2);
-  const int wgrp0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * ps.gsz;
   double* wsw = ws + (int64_t)w * L.per_walker;
   double acc_ltf = 0.0, acc_h0 = 0.0, acc_h1 = 0.0, acc_v0 = 0.0, acc_v1 = 0.0;
+  // schedule: both types present, or the one type t1
+  const bool two = s.ntypes > 1 && s.np[0] > 0 && s.np[1] > 0;
+  const int t1 = (s.ntypes > 1 && s.np[0] == 0) ? 1 : 0;
+  const int nU = two ? 128 : 32;
+  const int u0 = wv * nU / 4, u1 = (wv + 1) * nU / 4;
 
-  // pass 1: everything that needs one sweep over partners
-  for (int sl0 = 0; sl0 < ps.nslot; sl0 += ngrp) {
-    if (sl0 + wgrp0 >= ps.nslot) break;
-    const int ti = sl0 + wgrp0 >= ps.ns0 ? 1 : 0;  // wavefront-uniform
-    const int li = sl0 + grp - (ti ? ps.ns0 : 0);
-    const bool act = li < pick2(s.np, ti);
-    const int i = pick2(s.ipf, ti) + li;
-    double gx = 0, gy = 0, Gx = 0, Gy = 0, qx = 0, qy = 0, ltf = 0, hh = 0, vv = 0;
-    if (act) {
-      const double xi = xs[2 * i], yi = xs[2 * i + 1];
-      for (int tj = 0; tj < s.ntypes; tj++) {
+  // pass 1: u2, the pair term and G of u3, backflow q, pair potential
+  {
+    double ia[PP_NQ] = {0, 0, 0, 0, 0, 0};  // gx, gy, Gx, Gy, qx, qy of particle i
+    int icur = -1;
+    for (int u = u0; u < u1; u++) {
+      int ti, tj, d;
+      rr_unit(u, two, t1, ti, tj, d);
+      if (ti != icur) {
+        if (icur >= 0 && l < pick2(s.np, icur)) pp_add(part, N, pick2(s.ipf, icur) + l, ia, PP_NQ);
+#pragma unroll
+        for (int q = 0; q < PP_NQ; q++) ia[q] = 0.0;
+        icur = ti;
+      }
+      const int jl = (l + d) & 63;
+      if (l < pick2(s.np, ti) && jl < pick2(s.np, tj) && (ti != tj || d < 32 || l < 32)) {
+        const int i = pick2(s.ipf, ti) + l, j = pick2(s.ipf, tj) + jl;
         const int tp = tpair(ti, tj);
         const TabRef tu2 = tabref<TK_U2>(s, tp), tu3 = tabref<TK_U3>(s, tp),
                      tb = tabref<TK_B>(s, tp), tv = tabref<TK_V>(s, tp);
-        const int ve = pick3(s.vexp, tp);
-        const double vval = 0.5 * pick3(s.v2value, tp);
-        const int jb = pick2(s.ipf, tj), je = jb + pick2(s.np, tj);
-        for (int j = jb + lane; j < je; j += tpp) {
-          if (j == i) continue;
-          PairGeo p = pair_geo(s, xi, yi, xs[2 * j], xs[2 * j + 1]);
-          double f, df, ddf;
-          if (tu2.row0 >= 0) {
-            spline3(s, tu2, p.ind, p.rem, f, df, ddf);
-            ltf += 0.5 * f;
-            const double dtr = df * p.byr;
-            gx += dtr * p.ux;
-            gy += dtr * p.uy;
-            hh += ddf + dtr;
-          }
-          if (tu3.row0 >= 0) {
-            spline3(s, tu3, p.ind, p.rem, f, df, ddf);
-            Gx -= f * p.ux;
-            Gy -= f * p.uy;
-            // (xi r)^2 pair term of the three-body factor (reptation.f:2822-2832)
-            const double r = p.r;
-            const double tr = f * r;
-            const double d3 = 2.0 * f * (f + r * df);
-            const double dd3 = 2.0 * (f * f + 4.0 * r * f * df + r * r * (df * df + ddf * f));
-            ltf += 0.5 * tr * tr;
-            gx += d3 * p.ux;
-            gy += d3 * p.uy;
-            hh += dd3 + d3;
-          }
-          if (tb.row0 >= 0) {
-            spline3(s, tb, p.ind, p.rem, f, df, ddf);
-            qx += f * p.ux;
-            qy += f * p.uy;
-          }
-          if (tv.row0 >= 0) vv += vval * spline0(s, tv, ve, p.ind, p.rem, p.byr);
+        PairGeo p = pair_geo(s, xs[2 * i], xs[2 * i + 1], xs[2 * j], xs[2 * j + 1]);
+        double ja[PP_NQ] = {0, 0, 0, 0, 0, 0};
+        double hh = 0.0;
+        double f, df, ddf;
+        if (tu2.row0 >= 0) {
+          spline3(s, tu2, p.ind, p.rem, f, df, ddf);
+          acc_ltf += f;
+          const double dtr = df * p.byr;
+          ja[0] -= dtr * p.ux;
+          ja[1] -= dtr * p.uy;
+          hh += ddf + dtr;
         }
+        if (tu3.row0 >= 0) {
+          spline3(s, tu3, p.ind, p.rem, f, df, ddf);
+          ja[2] += f * p.ux;
+          ja[3] += f * p.uy;
+          // (xi r)^2 pair term of the three-body factor (reptation.f:2822-2832)
+          const double r = p.r;
+          const double tr = f * r;
+          const double d3 = 2.0 * f * (f + r * df);
+          const double dd3 = 2.0 * (f * f + 4.0 * r * f * df + r * r * (df * df + ddf * f));
+          acc_ltf += tr * tr;
+          ja[0] -= d3 * p.ux;
+          ja[1] -= d3 * p.uy;
+          hh += dd3 + d3;
+        }
+        if (tb.row0 >= 0) {
+          spline3(s, tb, p.ind, p.rem, f, df, ddf);
+          ja[4] -= f * p.ux;
+          ja[5] -= f * p.uy;
+        }
+        // the pair gives i the opposite of j's vector terms (u -> -u)
+#pragma unroll
+        for (int q = 0; q < PP_NQ; q++) ia[q] -= ja[q];
+        if (ti == 0) acc_h0 += hh; else acc_h1 += hh;
+        if (tj == 0) acc_h0 += hh; else acc_h1 += hh;
+        if (tv.row0 >= 0) {
+          const double vv = 0.5 * spline0(s, tv, pick3(s.vexp, tp), p.ind, p.rem, p.byr) * pick3(s.v2value, tp);
+          if (ti == 0) acc_v0 += vv; else acc_v1 += vv;
+          if (tj == 0) acc_v0 += vv; else acc_v1 += vv;
+        }
+        pp_add(part, N, j, ja, PP_NQ);
       }
     }
-    for (int o = 1; o < tpp; o <<= 1) {
-      gx += __shfl_xor(gx, o, 64);
-      gy += __shfl_xor(gy, o, 64);
-      Gx += __shfl_xor(Gx, o, 64);
-      Gy += __shfl_xor(Gy, o, 64);
-      qx += __shfl_xor(qx, o, 64);
-      qy += __shfl_xor(qy, o, 64);
-      ltf += __shfl_xor(ltf, o, 64);
-      hh += __shfl_xor(hh, o, 64);
-      vv += __shfl_xor(vv, o, 64);
-    }
-    if (act && lane == 0) {
-      Gs[2 * i] = Gx;
-      Gs[2 * i + 1] = Gy;
-      wsw[L.off_gj + 2 * i] = gx;
-      wsw[L.off_gj + 2 * i + 1] = gy;
-      wsw[L.off_q + 2 * i] = xs[2 * i] + qx;
-      wsw[L.off_q + 2 * i + 1] = xs[2 * i + 1] + qy;
-      acc_ltf += ltf - 0.5 * (Gx * Gx + Gy * Gy);
-      if (ti == 0) { acc_h0 += hh; acc_v0 += vv; }
-      else { acc_h1 += hh; acc_v1 += vv; }
-    }
+    if (icur >= 0 && l < pick2(s.np, icur)) pp_add(part, N, pick2(s.ipf, icur) + l, ia, PP_NQ);
+  }
+  __syncthreads();
+  // per-particle sums in a fixed order; Gs for pass 2, q out, g kept
+  double gpx = 0.0, gpy = 0.0;  // gradient of particle tid (N <= 128)
+  for (int p = tid; p < N; p += QMC_BLOCK) {
+    double v[PP_NQ];
+#pragma unroll
+    for (int q = 0; q < PP_NQ; q++)
+      v[q] = ((parts[(0 * PP_NQ + q) * N + p] + parts[(1 * PP_NQ + q) * N + p]) +
+              parts[(2 * PP_NQ + q) * N + p]) + parts[(3 * PP_NQ + q) * N + p];
+    gpx = v[0];
+    gpy = v[1];
+    Gs[2 * p] = v[2];
+    Gs[2 * p + 1] = v[3];
+    wsw[L.off_q + 2 * p] = xs[2 * p] + v[4];
+    wsw[L.off_q + 2 * p + 1] = xs[2 * p + 1] + v[5];
+    acc_ltf -= 0.5 * (v[2] * v[2] + v[3] * v[3]);
   }
   __syncthreads();
 
   // pass 2: three-body contraction (dotg, reptation.f:2928-2979), written per
   // particle i as g_i += sum_k M_ik (G_i - G_k),
   // h_i += -|sum_k M_ik|^2 - sum_k |M_ik|^2 + sum_k a_ik u_ik.(G_i - G_k)
-  // with M = xi I + (xi'/r) u u^T and a = xi'' + 3 xi'/r.
+  // with M = xi I + (xi'/r) u u^T and a = xi'' + 3 xi'/r; M and a.(u dG) are
+  // even in the pair, M dG is odd
   if (s.has_u3) {
-    for (int sl0 = 0; sl0 < ps.nslot; sl0 += ngrp) {
-      if (sl0 + wgrp0 >= ps.nslot) break;
-      const int ti = sl0 + wgrp0 >= ps.ns0 ? 1 : 0;
-      const int li = sl0 + grp - (ti ? ps.ns0 : 0);
-      const bool act = li < pick2(s.np, ti);
-      const int i = pick2(s.ipf, ti) + li;
-      double g3x = 0, g3y = 0, e11 = 0, e12 = 0, e22 = 0, nm2 = 0, h3 = 0;
-      if (act) {
-        const double xi = xs[2 * i], yi = xs[2 * i + 1];
-        const double Gix = Gs[2 * i], Giy = Gs[2 * i + 1];
-        for (int tj = 0; tj < s.ntypes; tj++) {
-          const TabRef tu3 = tabref<TK_U3>(s, tpair(ti, tj));
-          if (tu3.row0 < 0) continue;
-          const int jb = pick2(s.ipf, tj), je = jb + pick2(s.np, tj);
-          for (int j = jb + lane; j < je; j += tpp) {
-            if (j == i) continue;
-            PairGeo p = pair_geo(s, xi, yi, xs[2 * j], xs[2 * j + 1]);
-            double t, dt, ddt;
-            spline3(s, tu3, p.ind, p.rem, t, dt, ddt);
-            const double c = dt * p.byr;
-            const double m11 = t + c * p.ux * p.ux, m12 = c * p.ux * p.uy, m22 = t + c * p.uy * p.uy;
-            const double dGx = Gix - Gs[2 * j], dGy = Giy - Gs[2 * j + 1];
-            g3x += m11 * dGx + m12 * dGy;
-            g3y += m12 * dGx + m22 * dGy;
-            e11 += m11;
-            e12 += m12;
-            e22 += m22;
-            nm2 += m11 * m11 + 2.0 * m12 * m12 + m22 * m22;
-            h3 += (ddt + 3.0 * c) * (p.ux * dGx + p.uy * dGy);
-          }
-        }
+    for (int k = tid; k < 4 * PP_NQ * N; k += QMC_BLOCK) parts[k] = 0.0;
+    __syncthreads();
+    double ia[PP_NQ] = {0, 0, 0, 0, 0, 0};  // g3x, g3y, e11, e12, e22 of particle i
+    int icur = -1;
+    for (int u = u0; u < u1; u++) {
+      int ti, tj, d;
+      rr_unit(u, two, t1, ti, tj, d);
+      if (ti != icur) {
+        if (icur >= 0 && l < pick2(s.np, icur)) pp_add(part, N, pick2(s.ipf, icur) + l, ia, 5);
+#pragma unroll
+        for (int q = 0; q < PP_NQ; q++) ia[q] = 0.0;
+        icur = ti;
       }
-      for (int o = 1; o < tpp; o <<= 1) {
-        g3x += __shfl_xor(g3x, o, 64);
-        g3y += __shfl_xor(g3y, o, 64);
-        e11 += __shfl_xor(e11, o, 64);
-        e12 += __shfl_xor(e12, o, 64);
-        e22 += __shfl_xor(e22, o, 64);
-        nm2 += __shfl_xor(nm2, o, 64);
-        h3 += __shfl_xor(h3, o, 64);
-      }
-      if (act && lane == 0) {
-        h3 = h3 - (e11 * e11 + 2.0 * e12 * e12 + e22 * e22) - nm2;
-        wsw[L.off_gj + 2 * i] += g3x;
-        wsw[L.off_gj + 2 * i + 1] += g3y;
-        if (ti == 0) acc_h0 += h3;
-        else acc_h1 += h3;
+      const TabRef tu3 = tabref<TK_U3>(s, tpair(ti, tj));
+      const int jl = (l + d) & 63;
+      if (tu3.row0 >= 0 && l < pick2(s.np, ti) && jl < pick2(s.np, tj) && (ti != tj || d < 32 || l < 32)) {
+        const int i = pick2(s.ipf, ti) + l, j = pick2(s.ipf, tj) + jl;
+        PairGeo p = pair_geo(s, xs[2 * i], xs[2 * i + 1], xs[2 * j], xs[2 * j + 1]);
+        double t, dt, ddt;
+        spline3(s, tu3, p.ind, p.rem, t, dt, ddt);
+        const double c = dt * p.byr;
+        const double m11 = t + c * p.ux * p.ux, m12 = c * p.ux * p.uy, m22 = t + c * p.uy * p.uy;
+        const double dGx = Gs[2 * i] - Gs[2 * j], dGy = Gs[2 * i + 1] - Gs[2 * j + 1];
+        const double g3x = m11 * dGx + m12 * dGy, g3y = m12 * dGx + m22 * dGy;
+        const double nm2 = m11 * m11 + 2.0 * m12 * m12 + m22 * m22;
+        const double h3 = (ddt + 3.0 * c) * (p.ux * dGx + p.uy * dGy) - nm2;
+        ia[0] += g3x;
+        ia[1] += g3y;
+        ia[2] += m11;
+        ia[3] += m12;
+        ia[4] += m22;
+        const double ja[PP_NQ] = {-g3x, -g3y, m11, m12, m22, 0.0};
+        pp_add(part, N, j, ja, 5);
+        if (ti == 0) acc_h0 += h3; else acc_h1 += h3;
+        if (tj == 0) acc_h0 += h3; else acc_h1 += h3;
       }
     }
+    if (icur >= 0 && l < pick2(s.np, icur)) pp_add(part, N, pick2(s.ipf, icur) + l, ia, 5);
+    __syncthreads();
+    for (int p = tid; p < N; p += QMC_BLOCK) {
+      double v[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++)
+        v[q] = ((parts[(0 * PP_NQ + q) * N + p] + parts[(1 * PP_NQ + q) * N + p]) +
+                parts[(2 * PP_NQ + q) * N + p]) + parts[(3 * PP_NQ + q) * N + p];
+      gpx += v[0];
+      gpy += v[1];
+      const double e2 = v[2] * v[2] + 2.0 * v[3] * v[3] + v[4] * v[4];
+      if (ptype(s, p) == 0) acc_h0 -= e2; else acc_h1 -= e2;
+    }
+  }
+  for (int p = tid; p < N; p += QMC_BLOCK) {
+    wsw[L.off_gj + 2 * p] = gpx;
+    wsw[L.off_gj + 2 * p + 1] = gpy;
   }
 
   // rho_k per type and the k-space Ewald sum
