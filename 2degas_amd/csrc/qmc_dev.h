@@ -18,6 +18,16 @@
 // Type pair index of particles of types ti, tj (0: 1-1, 1: 1-2, 2: 2-2).
 __host__ __device__ inline int tpair(int ti, int tj) { return ti + tj; }
 
+// Per type pair: the first row and grid size of its u2, u3, backflow and
+// potential tables (row -1: no table), the potential's 1/r exponent and its
+// weight.  A copy lives in device memory so that a loop over type pairs
+// reads it with scalar loads instead of keeping every pair's values live.
+struct __attribute__((aligned(16))) TpInfo {
+  int row[4], ng[4];
+  int vexp, pad;
+  double v2value;
+};
+
 struct DevSys {
   int ntypes, nptot, npnorm;
   int np[2], ipf[2];
@@ -35,6 +45,7 @@ struct DevSys {
   int trow[4][3], tng[4][3], vexp[3];
   int iub_flag[2][2];      // determinant of spin s couples particle type t
   double v2value[3];
+  const TpInfo* tpi;       // [3], the same tables per type pair
   double tailc[2], tailc_tot;  // pair-potential tail constants
   // k space
   int nk, nrhok, irhok[2];

@@ -38,6 +38,7 @@ struct qmc_ctx {
   double* d_gvec = nullptr;
   double* d_kxyv = nullptr;  // distinct kx then ky components
   int* d_kxy = nullptr;
+  TpInfo* d_tpi = nullptr;
   // workspace sized for ws_nw walkers
   int ws_nw = 0;
   double* d_ws = nullptr;

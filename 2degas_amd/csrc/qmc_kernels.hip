@@ -97,9 +97,9 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
       const int jl = (l + d) & 63;
       if (l < pick2(s.np, ti) && jl < pick2(s.np, tj) && (ti != tj || d < 32 || l < 32)) {
         const int i = pick2(s.ipf, ti) + l, j = pick2(s.ipf, tj) + jl;
-        const int tp = tpair(ti, tj);
-        const TabRef tu2 = tabref<TK_U2>(s, tp), tu3 = tabref<TK_U3>(s, tp),
-                     tb = tabref<TK_B>(s, tp), tv = tabref<TK_V>(s, tp);
+        const TpInfo* T = s.tpi + tpair(ti, tj);
+        const TabRef tu2{T->row[TK_U2], T->ng[TK_U2]}, tu3{T->row[TK_U3], T->ng[TK_U3]},
+                     tb{T->row[TK_B], T->ng[TK_B]}, tv{T->row[TK_V], T->ng[TK_V]};
         PairGeo p = pair_geo(s, xs[2 * i], xs[2 * i + 1], xs[2 * j], xs[2 * j + 1]);
         double ja[PP_NQ] = {0, 0, 0, 0, 0, 0};
         double hh = 0.0;
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
         if (ti == 0) acc_h0 += hh; else acc_h1 += hh;
         if (tj == 0) acc_h0 += hh; else acc_h1 += hh;
         if (tv.row0 >= 0) {
-          const double vv = 0.5 * spline0(s, tv, pick3(s.vexp, tp), p.ind, p.rem, p.byr) * pick3(s.v2value, tp);
+          const double vv = 0.5 * spline0(s, tv, T->vexp, p.ind, p.rem, p.byr) * T->v2value;
           if (ti == 0) acc_v0 += vv; else acc_v1 += vv;
           if (tj == 0) acc_v0 += vv; else acc_v1 += vv;
         }
@@ -184,7 +184,8 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
         for (int q = 0; q < PP_NQ; q++) ia[q] = 0.0;
         icur = ti;
       }
-      const TabRef tu3 = tabref<TK_U3>(s, tpair(ti, tj));
+      const TpInfo* T = s.tpi + tpair(ti, tj);
+      const TabRef tu3{T->row[TK_U3], T->ng[TK_U3]};
       const int jl = (l + d) & 63;
       if (tu3.row0 >= 0 && l < pick2(s.np, ti) && jl < pick2(s.np, tj) && (ti != tj || d < 32 || l < 32)) {
         const int i = pick2(s.ipf, ti) + l, j = pick2(s.ipf, tj) + jl;
@@ -239,11 +240,15 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
       // ix | iy << 8 | k << 16, sorted by ix then iy), so the threads of a
       // wavefront share kx phases and read consecutive ky phases from LDS.
       int kq[KMAXPT];
+      int ox[KMAXPT], oy[KMAXPT];  // kx and ky phase slots of slot q (0 when none)
 #pragma unroll
       for (int q = 0; q < KMAXPT; q++) {
         rc0[q] = rc1[q] = rs0[q] = rs1[q] = 0.0;
         const int n = kb + threadIdx.x + q * QMC_BLOCK;
-        kq[q] = n >= s.nk ? -1 : (s.nkxv > 0 ? (s.kxy[n] >> 16) : n);
+        const int code = (n < s.nk && s.nkxv > 0) ? s.kxy[n] : 0;
+        kq[q] = n >= s.nk ? -1 : (s.nkxv > 0 ? (code >> 16) : n);
+        ox[q] = code & 0xff;
+        oy[q] = n < s.nk ? ((code >> 8) & 0xff) + s.nkxv : 0;
       }
       if (s.nkxv > 0) {
         // e^{i k.x} = e^{i kx x} e^{i ky y}: per particle chunk, the distinct
@@ -262,22 +267,25 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
             ph[e] = make_double2(cv, sv);
           }
           __syncthreads();
+          // particles of type 0 come first: [0, split) of the chunk; the
+          // phase loads of two k-vectors are issued before their products
           const int split = s.ntypes > 1 ? min(max(s.ipf[1] - c0, 0), cn) : cn;
+          for (int pc = 0; pc < cn; pc++) {
+            const double2* row = ph + pc * nv;
 #pragma unroll
-          for (int q = 0; q < KMAXPT; q++) {
-            if (kq[q] < 0) continue;
-            const int code = s.kxy[kb + threadIdx.x + q * QMC_BLOCK];
-            const int ix = code & 0xff, iy = ((code >> 8) & 0xff) + s.nkxv;
-            // particles of type 0 come first: [0, split) of the chunk
-            for (int pc = 0; pc < split; pc++) {
-              const double2 a = ph[pc * nv + ix], bb = ph[pc * nv + iy];
-              rc0[q] += a.x * bb.x - a.y * bb.y;
-              rs0[q] += a.x * bb.y + a.y * bb.x;
-            }
-            for (int pc = split; pc < cn; pc++) {
-              const double2 a = ph[pc * nv + ix], bb = ph[pc * nv + iy];
-              rc1[q] += a.x * bb.x - a.y * bb.y;
-              rs1[q] += a.x * bb.y + a.y * bb.x;
+            for (int h = 0; h < KMAXPT; h += 2) {
+              const double2 a0 = row[ox[h]], b0 = row[oy[h]], a1 = row[ox[h + 1]], b1 = row[oy[h + 1]];
+              if (pc < split) {
+                rc0[h] += a0.x * b0.x - a0.y * b0.y;
+                rs0[h] += a0.x * b0.y + a0.y * b0.x;
+                rc0[h + 1] += a1.x * b1.x - a1.y * b1.y;
+                rs0[h + 1] += a1.x * b1.y + a1.y * b1.x;
+              } else {
+                rc1[h] += a0.x * b0.x - a0.y * b0.y;
+                rs1[h] += a0.x * b0.y + a0.y * b0.x;
+                rc1[h + 1] += a1.x * b1.x - a1.y * b1.y;
+                rs1[h + 1] += a1.x * b1.y + a1.y * b1.x;
+              }
             }
           }
         }
