@@ -80,15 +80,49 @@ __device__ __forceinline__ void xc_contract(const xc_d4 (&acc)[2][XC_TPW], const
   }
 }
 
-// Per-wavefront body for tile group (G, H): every chunk of V_t is staged by
-// the whole workgroup, then acc += V V^T on this wavefront's tiles.
+// Entry e (row j = e % 64, particle pc = e / 64) of the V chunk of
+// particles c0..c0+cn-1: rows j beyond m and particles beyond cn are zero;
+// the 2 x 2 block of (j, ip) is symmetric.  Consecutive threads take
+// consecutive rows, so the column-major stores spread over the LDS banks.
+__device__ __forceinline__ void xc_entry(const DevSys& s, const double* __restrict__ Qd,
+                                         const double* __restrict__ xs, double* __restrict__ Vt, TabRef tb,
+                                         int e, int c0, int cn, int m, int p0) {
+  const int j = e & 63, pc = e >> 6;
+  double v11 = 0, v12 = 0, v22 = 0;
+  const int ip = c0 + pc;
+  if (j < m && pc < cn) {
+    const int jp = p0 + j;
+    if (jp == ip) {
+      v11 = 1.0 + Qd[3 * j];
+      v12 = Qd[3 * j + 1];
+      v22 = 1.0 + Qd[3 * j + 2];
+    } else if (tb.row0 >= 0) {
+      PairGeo pg = pair_geo(s, xs[2 * jp], xs[2 * jp + 1], xs[2 * ip], xs[2 * ip + 1]);
+      double eta, deta, ddeta;
+      spline3(s, tb, pg.ind, pg.rem, eta, deta, ddeta);
+      const double cb = deta * pg.byr;
+      v11 = -(eta + cb * pg.ux * pg.ux);
+      v12 = -(cb * pg.ux * pg.uy);
+      v22 = -(eta + cb * pg.uy * pg.uy);
+    }
+  }
+  Vt[(2 * pc) * XC_RS + 2 * j] = v11;
+  Vt[(2 * pc + 1) * XC_RS + 2 * j] = v12;
+  Vt[(2 * pc) * XC_RS + 2 * j + 1] = v12;
+  Vt[(2 * pc + 1) * XC_RS + 2 * j + 1] = v22;
+}
+
+// Per-wavefront body for tile group (G, H).  The V chunks (16 particles of
+// one type at a time, over the types with backflow to this spin) are
+// double-buffered: while the matrix cores take this chunk's products the
+// vector units build the next chunk into the other buffer (two entries per
+// thread, placed between the MFMA steps), so a chunk costs one barrier.
 template <int G, int H>
 __device__ __forceinline__ void xc_body(const DevSys& s, double* __restrict__ dsc, const double* __restrict__ Qd,
                                         const zd* __restrict__ Fg, const double* __restrict__ xs,
                                         double* __restrict__ Vt, double* red, int sp, int m, int p0, int tid) {
   const int lane = tid & 63;
   const int nT = (2 * m + 15) >> 4;
-  const double* base = Vt + (lane >> 4) * XC_RS + (lane & 15);
   // S_t accumulators of both particle types (zero for a type without
   // backflow to this spin), contracted with M in one pass over F at the end
   xc_d4 acc[2][XC_TPW];
@@ -96,44 +130,33 @@ __device__ __forceinline__ void xc_body(const DevSys& s, double* __restrict__ ds
   for (int t = 0; t < 2; t++)
 #pragma unroll
     for (int q = 0; q < XC_TPW; q++) acc[t][q] = xc_d4{0.0, 0.0, 0.0, 0.0};
+  bool act[2];
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+    act[t] = t < s.ntypes && (sp ? pick2(s.iub_flag[1], t) : pick2(s.iub_flag[0], t)) && pick2(s.np, t) > 0;
+  const int ft = act[0] ? 0 : (act[1] ? 1 : -1);
+  if (ft >= 0) {
+    const int c0 = pick2(s.ipf, ft), cn = min(XC_KC / 2, pick2(s.np, ft));
+    const TabRef tb = tabref<TK_B>(s, tpair(sp, ft));
+    for (int e = tid; e < 64 * (XC_KC / 2); e += XC_THREADS) xc_entry(s, Qd, xs, Vt, tb, e, c0, cn, m, p0);
+  }
+  __syncthreads();
+  int cur = 0;
 #pragma unroll
   for (int t = 0; t < 2; t++) {
-    if (t >= s.ntypes) break;
-    if (!(sp ? pick2(s.iub_flag[1], t) : pick2(s.iub_flag[0], t)) || pick2(s.np, t) == 0) continue;
+    if (!act[t]) continue;
     const int ia = pick2(s.ipf, t), ib = ia + pick2(s.np, t);
-    const TabRef tb = tabref<TK_B>(s, tpair(sp, t));
     for (int c0 = ia; c0 < ib; c0 += XC_KC / 2) {
-      const int cn = min(XC_KC / 2, ib - c0);
-      // V chunk: 64 particle rows j (zero beyond m) x 16 particles (zero
-      // beyond cn); the 2 x 2 block of (j, ip) is symmetric
-      for (int e = tid; e < 64 * (XC_KC / 2); e += XC_THREADS) {
-        // consecutive threads take consecutive rows: the column-major writes
-        // then spread over the LDS banks
-        const int j = e & 63, pc = e >> 6;
-        double v11 = 0, v12 = 0, v22 = 0;
-        const int ip = c0 + pc;
-        if (j < m && pc < cn) {
-          const int jp = p0 + j;
-          if (jp == ip) {
-            v11 = 1.0 + Qd[3 * j];
-            v12 = Qd[3 * j + 1];
-            v22 = 1.0 + Qd[3 * j + 2];
-          } else if (tb.row0 >= 0) {
-            PairGeo pg = pair_geo(s, xs[2 * jp], xs[2 * jp + 1], xs[2 * ip], xs[2 * ip + 1]);
-            double eta, deta, ddeta;
-            spline3(s, tb, pg.ind, pg.rem, eta, deta, ddeta);
-            const double cb = deta * pg.byr;
-            v11 = -(eta + cb * pg.ux * pg.ux);
-            v12 = -(cb * pg.ux * pg.uy);
-            v22 = -(eta + cb * pg.uy * pg.uy);
-          }
-        }
-        Vt[(2 * pc) * XC_RS + 2 * j] = v11;
-        Vt[(2 * pc + 1) * XC_RS + 2 * j] = v12;
-        Vt[(2 * pc) * XC_RS + 2 * j + 1] = v12;
-        Vt[(2 * pc + 1) * XC_RS + 2 * j + 1] = v22;
+      // the next chunk: the rest of this type, then the other type
+      int nt = t, nc0 = c0 + XC_KC / 2;
+      if (nc0 >= ib) {
+        nt = (t == 0 && act[1]) ? 1 : -1;
+        nc0 = nt >= 0 ? pick2(s.ipf, 1) : 0;
       }
-      __syncthreads();
+      const int ncn = nt >= 0 ? min(XC_KC / 2, pick2(s.ipf, nt) + pick2(s.np, nt) - nc0) : 0;
+      const TabRef ntb = nt >= 0 ? tabref<TK_B>(s, tpair(sp, nt)) : TabRef{-1, 0};
+      double* Vn = Vt + (cur ^ 1) * (XC_KC * XC_RS);
+      const double* base = Vt + cur * (XC_KC * XC_RS) + (lane >> 4) * XC_RS + (lane & 15);
 #pragma unroll
       for (int ks = 0; ks < XC_KC / 4; ks++) {
         // rows beyond 2m are zero in LDS, so tiles past the last block row
@@ -149,10 +172,13 @@ __device__ __forceinline__ void xc_body(const DevSys& s, double* __restrict__ ds
         }
 #pragma unroll
         for (int q = 0; q < xc_nt(H); q++) xc_pin(acc[t][q]);
+        if (nt >= 0 && (ks == 1 || ks == 5))
+          xc_entry(s, Qd, xs, Vn, ntb, tid + (ks == 5 ? XC_THREADS : 0), nc0, ncn, m, p0);
         // the next step's operands are read after this step's MFMAs issue
         __asm__ volatile("" ::: "memory");
       }
       __syncthreads();
+      cur ^= 1;
     }
   }
   double u0, u1;
@@ -165,8 +191,8 @@ __device__ __forceinline__ void xc_body(const DevSys& s, double* __restrict__ ds
   }
 }
 
-// LDS: xs[2N], then Vt[XC_KC][XC_RS] (column = (particle, idim) of the
-// chunk, row = (j, a)).
+// LDS: xs[2N], then two Vt[XC_KC][XC_RS] chunk buffers (column = (particle,
+// idim) of the chunk, row = (j, a)).
 __global__ __launch_bounds__(XC_THREADS, 4) void k_cross(DevSys s, WsLayout L, const double* __restrict__ x,
                                                          double* __restrict__ ws,
                                                          const double* __restrict__ Fbuf) {
