@@ -55,7 +55,8 @@ struct DevSys {
   int nkxv, nkyv, kchunk;
   const double* kxv;
   const double* kyv;
-  const int* kxy;          // [nk] kxi | kyi << 16
+  const int* kxy;          // [nkslot] factorized rho_k slots: kxi | kyi << 8 | (k + 1) << 16
+  int nkslot;
   int nke[3];              // nk_ewald of the v2 table of each type pair
   const double* vk[3];     // [nke]
   double vk0[3];

@@ -232,23 +232,20 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
   double pk0 = 0.0, pk1 = 0.0, pkt = 0.0;
   if (s.nrhok) {
     double* pw = props + (int64_t)w * s.n_props;
-    // passes of KMAXPT k-vectors per thread
-    for (int kb = 0; kb < s.nk; kb += KMAXPT * QMC_BLOCK) {
+    // passes of KMAXPT slots per thread: slots q = 2h, 2h+1 of a thread are
+    // the two k-vectors of one item, which share their ky component
+    for (int kb = 0; kb < s.nkslot; kb += KMAXPT * QMC_BLOCK) {
       double rc0[KMAXPT], rc1[KMAXPT], rs0[KMAXPT], rs1[KMAXPT];
-      // k-vector of slot q of this thread (-1: none).  The factorized path
-      // visits the k-vectors in (kx, ky) component order (kxy entries hold
-      // ix | iy << 8 | k << 16, sorted by ix then iy), so the threads of a
-      // wavefront share kx phases and read consecutive ky phases from LDS.
-      int kq[KMAXPT];
+      int kq[KMAXPT];              // k-vector of slot q (-1: none)
       int ox[KMAXPT], oy[KMAXPT];  // kx and ky phase slots of slot q (0 when none)
 #pragma unroll
       for (int q = 0; q < KMAXPT; q++) {
         rc0[q] = rc1[q] = rs0[q] = rs1[q] = 0.0;
-        const int n = kb + threadIdx.x + q * QMC_BLOCK;
-        const int code = (n < s.nk && s.nkxv > 0) ? s.kxy[n] : 0;
-        kq[q] = n >= s.nk ? -1 : (s.nkxv > 0 ? (code >> 16) : n);
+        const int n = kb + 2 * (threadIdx.x + (q >> 1) * QMC_BLOCK) + (q & 1);
+        const int code = (n < s.nkslot && s.nkxv > 0) ? s.kxy[n] : 0;
+        kq[q] = n >= s.nkslot ? -1 : (s.nkxv > 0 ? (code >> 16) - 1 : n);
         ox[q] = code & 0xff;
-        oy[q] = n < s.nk ? ((code >> 8) & 0xff) + s.nkxv : 0;
+        oy[q] = ((code >> 8) & 0xff) + s.nkxv;
       }
       if (s.nkxv > 0) {
         // e^{i k.x} = e^{i kx x} e^{i ky y}: per particle chunk, the distinct
@@ -274,17 +271,18 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
             const double2* row = ph + pc * nv;
 #pragma unroll
             for (int h = 0; h < KMAXPT; h += 2) {
-              const double2 a0 = row[ox[h]], b0 = row[oy[h]], a1 = row[ox[h + 1]], b1 = row[oy[h + 1]];
+              // one ky phase for the item's two kx phases
+              const double2 b = row[oy[h]], a0 = row[ox[h]], a1 = row[ox[h + 1]];
               if (pc < split) {
-                rc0[h] += a0.x * b0.x - a0.y * b0.y;
-                rs0[h] += a0.x * b0.y + a0.y * b0.x;
-                rc0[h + 1] += a1.x * b1.x - a1.y * b1.y;
-                rs0[h + 1] += a1.x * b1.y + a1.y * b1.x;
+                rc0[h] = fma(a0.x, b.x, fma(-a0.y, b.y, rc0[h]));
+                rs0[h] = fma(a0.x, b.y, fma(a0.y, b.x, rs0[h]));
+                rc0[h + 1] = fma(a1.x, b.x, fma(-a1.y, b.y, rc0[h + 1]));
+                rs0[h + 1] = fma(a1.x, b.y, fma(a1.y, b.x, rs0[h + 1]));
               } else {
-                rc1[h] += a0.x * b0.x - a0.y * b0.y;
-                rs1[h] += a0.x * b0.y + a0.y * b0.x;
-                rc1[h + 1] += a1.x * b1.x - a1.y * b1.y;
-                rs1[h + 1] += a1.x * b1.y + a1.y * b1.x;
+                rc1[h] = fma(a0.x, b.x, fma(-a0.y, b.y, rc1[h]));
+                rs1[h] = fma(a0.x, b.y, fma(a0.y, b.x, rs1[h]));
+                rc1[h + 1] = fma(a1.x, b.x, fma(-a1.y, b.y, rc1[h + 1]));
+                rs1[h + 1] = fma(a1.x, b.y, fma(a1.y, b.x, rs1[h + 1]));
               }
             }
           }
