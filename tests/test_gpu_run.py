@@ -1,0 +1,165 @@
+"""The batched VMC driver (2degas_amd/run.py) against the reference's own
+driver on the CPU (oracle/_ref/refqmc, built from the reference sources with
+the single-rank MPI stub), n26g deck, 0.05 time step:
+
+* one walker = one rank: the stdout block lines, `n26.res`, the rhok files,
+  `<x_file>.res.0` and `seed` agree (numbers to 1e-8 relative, names,
+  block counts and generator limbs exactly), then a `restart` record
+  continues both runs for one more block;
+* three walkers = three ranks: rank k of the batch is a single-rank
+  reference run whose seed's last limb is offset by k (the rank rule
+  2 (s4 + k) + 1, reptation.f:37-38) started from configuration file k; the
+  batch's block means are the mean of the three runs' block means, and
+  each rank's generator limbs and final configuration are that run's."""
+import io
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import DECKS, ROOT, pkg
+
+pytestmark = pytest.mark.gpu
+REF = os.path.join(ROOT, "oracle", "_ref", "refqmc")
+NEEDS_REF = pytest.mark.skipif(not os.path.exists(REF),
+                               reason="reference driver not built (needs the reference sources)")
+TYPES = ("up", "down")
+
+
+def _deck_dir(work, seed=None, configs=None):
+    shutil.copytree(os.path.join(DECKS, "n26g"), work, copy_function=shutil.copyfile)
+    for t in TYPES:
+        shutil.copyfile(os.path.join(work, f"n26.{t}.x.res.0"), os.path.join(work, f"n26.{t}.x.0"))
+        os.remove(os.path.join(work, f"n26.{t}.x.res.0"))
+    if seed is not None:
+        with open(os.path.join(work, "seed"), "w") as f:
+            f.write(" " + " ".join(str(v) for v in seed) + "\n")
+    for k, x in (configs or {}).items():
+        off = 0
+        for t in TYPES:
+            with open(os.path.join(work, f"n26.{t}.x.{k}"), "w") as f:
+                for p in x[off:off + 13]:
+                    f.write(f" {float(p[0])!r} {float(p[1])!r}\n")
+            off += 13
+
+
+def _script(work, text):
+    with open(os.path.join(work, "n26.in"), "w") as f:
+        f.write(text)
+
+
+def _ref(work):
+    r = subprocess.run([REF], cwd=work, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=300)
+    assert r.returncode == 0, r.stdout.decode()[-2000:]
+    return r.stdout.decode()
+
+
+def _ours(work, nw):
+    out = io.StringIO()
+    pkg("run").run(work, nw, out=out)
+    return out.getvalue()
+
+
+def _blocks(text):
+    """{block: [(values..., name)]} from the block lines of stdout."""
+    blocks, cur = {}, None
+    for line in text.splitlines():
+        if "===>> vmc block" in line:
+            cur = int(line.split()[-1])
+            blocks[cur] = []
+        elif cur is not None:
+            try:
+                vals = [float(line[0:19]), float(line[19:38]), float(line[38:47]), float(line[47:57])]
+            except ValueError:
+                cur = None
+                continue
+            blocks[cur].append((vals, line[59:].strip()))
+    return blocks
+
+
+def _res(work):
+    with open(os.path.join(work, "n26.res")) as f:
+        tok = f.read().split()
+    return tok
+
+
+def _close(a, b, rtol, what):
+    a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
+    den = np.maximum(np.abs(b), 1e-300)
+    err = np.max(np.abs(a - b) / np.where(np.abs(b) > 1e-12, den, 1.0))
+    assert err <= rtol, f"{what}: {err:.2e}"
+    return err
+
+
+@NEEDS_REF
+def test_one_walker_reproduces_reference_driver_files(tmp_path):
+    print("\n[gpu] batched driver vs reference driver, 1 rank", flush=True)
+    ref, ours = str(tmp_path / "ref"), str(tmp_path / "ours")
+    for w in (ref, ours):
+        _deck_dir(w)
+        _script(w, "vmc 2 20 0.05 0\nend\n")
+    out_r, out_o = _ref(ref), _ours(ours, 1)
+    br, bo = _blocks(out_r), _blocks(out_o)
+    assert sorted(br) == sorted(bo) == [1, 2]
+    for b in br:
+        assert [n for _, n in br[b]] == [n for _, n in bo[b]]
+        _close([v for v, _ in bo[b]], [v for v, _ in br[b]], 1e-8, f"block {b}")
+    tr, to = _res(ref), _res(ours)
+    assert tr[:2] == to[:2] and len(tr) == len(to)
+    # cumulative sums, then the 8 generator limbs of the one rank
+    _close([float(v) for v in to[2:-8]], [float(v) for v in tr[2:-8]], 1e-8, "restart sums")
+    assert tr[-8:] == to[-8:]
+    for name in ("n26.rhok_up", "n26.rhok_down"):
+        a = np.loadtxt(os.path.join(ours, name), usecols=(0, 1, 2, 3, 5))
+        b = np.loadtxt(os.path.join(ref, name), usecols=(0, 1, 2, 3, 5))
+        assert a.shape == b.shape
+        # the error column of rhok is a difference of nearly equal sums
+        _close(a[:, [0, 1, 3, 4]], b[:, [0, 1, 3, 4]], 1e-8, name)
+    for t in TYPES:
+        a = np.loadtxt(os.path.join(ours, f"n26.{t}.x.res.0"))
+        b = np.loadtxt(os.path.join(ref, f"n26.{t}.x.res.0"))
+        _close(a, b, 1e-10, f"{t} configuration")
+    with open(os.path.join(ref, "seed")) as f, open(os.path.join(ours, "seed")) as g:
+        assert f.read().split() == g.read().split()
+    # restart: one more block from the files both runs wrote
+    for w in (ref, ours):
+        _script(w, "restart 1\nvmc 3 20 0.05 0\nend\n")
+    out_r, out_o = _ref(ref), _ours(ours, 1)
+    br, bo = _blocks(out_r), _blocks(out_o)
+    assert sorted(br) == sorted(bo) == [3]
+    _close([v for v, _ in bo[3]], [v for v, _ in br[3]], 1e-8, "block 3 after restart")
+    assert _res(ref)[-8:] == _res(ours)[-8:]
+
+
+@NEEDS_REF
+def test_walkers_are_ranks(tmp_path):
+    print("\n[gpu] batched driver, 3 walkers vs 3 single-rank reference runs", flush=True)
+    dg = pkg("deckgen")
+    d = pkg("deck").read_deck(os.path.join(DECKS, "n26g"))
+    x = dg.lattice_configs(3, d.nptot, d.el[0], 4242)
+    seed = [3535, 3467, 3262, 1549]
+    ours = str(tmp_path / "ours")
+    _deck_dir(ours, seed, {k: x[k] for k in range(3)})
+    _script(ours, "vmc 2 20 0.05 0\nend\n")
+    bo = _blocks(_ours(ours, 3))
+    means = {b: [] for b in (1, 2)}
+    tok = _res(ours)
+    limbs = tok[-24:]
+    for k in range(3):
+        ref = str(tmp_path / f"ref{k}")
+        _deck_dir(ref, seed[:3] + [seed[3] + k], {0: x[k]})
+        _script(ref, "vmc 2 20 0.05 0\nend\n")
+        br = _blocks(_ref(ref))
+        for b in (1, 2):
+            means[b].append([v[0] for v, _ in br[b]])
+        # the primary generator of rank k; its second generator is the base seed
+        assert _res(ref)[-8:-4] == limbs[8 * k:8 * k + 4]
+        assert limbs[8 * k + 4:8 * k + 8] == [str(v) for v in seed]
+        for t in TYPES:
+            a = np.loadtxt(os.path.join(ours, f"n26.{t}.x.res.{k}"))
+            b = np.loadtxt(os.path.join(ref, f"n26.{t}.x.res.0"))
+            _close(a, b, 1e-10, f"rank {k} {t} configuration")
+    for b in (1, 2):
+        _close([v[0] for v, _ in bo[b]], np.mean(np.array(means[b]), axis=0), 1e-8, f"block {b} means")
