@@ -19,9 +19,11 @@ MPI rank k of the reference:
   reptation.f:4510-4536): the next block, the cumulative sums, every rank's
   generator state and, with twists, the current twist;
 * at the end rank 0's generator state goes to `seed` (reptation.f:844-851);
+* at the end of the command every rank's configuration goes back to
+  `<x_file>.<k>` (write_conf, reptation.f:1008-1011, 2503-2535);
 * a deck with `ntwists n` runs the blocks once per twist (ith, jth) in
-  1..n x 1..n, as tabc does (reptation.f:9559-9638), each twist starting
-  from the configuration files again.
+  1..n x 1..n, as tabc does (reptation.f:9559-9638), each twist reading
+  the configurations the previous one wrote.
 
 Block averages sum p_old over the walkers and steps of the block, the
 MPI_REDUCE of the reference (reptation.f:4308-4311), so that blk_norm is
@@ -168,13 +170,15 @@ class Run:
             x.append(deckmod.read_configs(paths, d.np, 1)[0])
         return np.stack(x)
 
-    def write_configs(self, x):
+    def write_configs(self, x, mid=".res."):
+        """`<x_file><mid><k>`: `.res.` after every block (restart(1)), `.`
+        at the end of a vmc command (write_conf, reptation.f:2503-2535)."""
         d = self.deck
         off = 0
         for it, f in enumerate(d.x_file):
             n = d.np[it]
             for k in range(self.nw):
-                with open(self._path(f"{f}.res.{k}"), "w") as fh:
+                with open(self._path(f"{f}{mid}{k}"), "w") as fh:
                     for p in x[k, off:off + n]:
                         fh.write(f" {_ld(p[0])} {_ld(p[1])}\n")
             off += n
@@ -282,6 +286,12 @@ class Run:
                     self.block_output(iblk, blk, blk_norm, names, info)
                     self.write_restart(iblk)
                     self.write_configs(wk.x.cpu().numpy())
+                # the end of vmc: the configurations go back to the files the
+                # next command (or twist) reads, and after a restart the
+                # following twists start at block 1 (reptation.f:1008-1015)
+                self.write_configs(wk.x.cpu().numpy(), ".")
+                if d.ntheta != 0 and self.res_string != ".":
+                    self.iblk0 = 1
                 wk = None
             finally:
                 ctx.close()
