@@ -16,7 +16,8 @@
 // particles (32 columns) at a time, column-major so each MFMA operand is one
 // 8-byte LDS read per lane.  M is formed from F (L2) tile by tile when S_t is
 // complete, once for both particle types, so only the S accumulators stay in
-// registers through the chunks and F is read once per (walker, spin).
+// registers through the chunks and F is read once per (walker, spin); it is
+// prefetched into L2 while the last chunk's products run.
 #include "qmc_dev.h"
 
 typedef double xc_d4 __attribute__((ext_vector_type(4)));
@@ -112,6 +113,9 @@ __device__ __forceinline__ void xc_entry(const DevSys& s, const double* __restri
   Vt[(2 * pc + 1) * XC_RS + 2 * j + 1] = v22;
 }
 
+// Destination of the L2 prefetch of F: one wavefront's 64 x 16 bytes, never read.
+__shared__ __attribute__((aligned(16))) unsigned xc_sink[256];
+
 // Per-wavefront body for tile group (G, H).  The V chunks (16 particles of
 // one type at a time, over the types with backflow to this spin) are
 // double-buffered: while the matrix cores take this chunk's products the
@@ -174,6 +178,16 @@ __device__ __forceinline__ void xc_body(const DevSys& s, double* __restrict__ ds
         for (int q = 0; q < xc_nt(H); q++) xc_pin(acc[t][q]);
         if (nt >= 0 && (ks == 1 || ks == 5))
           xc_entry(s, Qd, xs, Vn, ntb, tid + (ks == 5 ? XC_THREADS : 0), nc0, ncn, m, p0);
+        // during the last chunk, F of this (walker, spin) is fetched into
+        // the L2 cache (16 bytes a lane, into an LDS sink nothing reads), a
+        // quarter at every other step, so the contraction below reads it
+        // from there instead of waiting on HBM tile by tile
+        if (nt < 0 && (ks & 1)) {
+          const int nz = 2 * m * m;
+          for (int i = tid + (ks >> 1) * XC_THREADS; i < nz; i += 4 * XC_THREADS)
+            __builtin_amdgcn_global_load_lds((const void*)(Fg + i), (void __attribute__((address_space(3)))*)xc_sink,
+                                             16, 0, 0);
+        }
         // the next step's operands are read after this step's MFMAs issue
         __asm__ volatile("" ::: "memory");
       }
@@ -198,7 +212,9 @@ __global__ __launch_bounds__(XC_THREADS, 4) void k_cross(DevSys s, WsLayout L, c
                                                          const double* __restrict__ Fbuf) {
   extern __shared__ double sm[];
   __shared__ double red[XC_THREADS / 64];
-  const int w = blockIdx.x, sp = blockIdx.y;
+  // walkers and spins in the reverse of the order in which k_gjm wrote their
+  // F, so the first workgroups find theirs still in the memory-side cache
+  const int w = gridDim.x - 1 - blockIdx.x, sp = gridDim.y - 1 - blockIdx.y;
   const int m = pick2(s.np, sp);
   if (m == 0) return;
   const int N = s.nptot, p0 = pick2(s.ipf, sp);
