@@ -184,6 +184,9 @@ __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, dou
   for (int k0 = 0; k0 < m; k0 += 4) {
     const int nb = min(4, m - k0);
     if (wave == ((k0 >> 2) & 3)) {
+      // the pivot steps are the chain every wavefront of the workgroup waits
+      // on: they issue ahead of the other workgroup's products on this SIMD
+      __builtin_amdgcn_s_setprio(1);
       const int sw = gm_swz(lane);
       zd P[4], Fp[4];
 #pragma unroll
@@ -205,6 +208,7 @@ __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, dou
       }
 #pragma unroll
       for (int j = 0; j < 4; j++) sh.Fl[lane * GM_PS + j] = Fp[j];
+      __builtin_amdgcn_s_setprio(0);
     }
     __syncthreads();
     int pb[4];
