@@ -161,21 +161,26 @@ __device__ __forceinline__ void xc_body(const DevSys& s, double* __restrict__ ds
       const TabRef ntb = nt >= 0 ? tabref<TK_B>(s, tpair(sp, nt)) : TabRef{-1, 0};
       double* Vn = Vt + (cur ^ 1) * (XC_KC * XC_RS);
       const double* base = Vt + cur * (XC_KC * XC_RS) + (lane >> 4) * XC_RS + (lane & 15);
+      // particles in this chunk: its columns 2 cn and beyond are zero, and
+      // a step whose four columns all are is skipped
+      const int cn = min(XC_KC / 2, ib - c0);
 #pragma unroll
       for (int ks = 0; ks < XC_KC / 4; ks++) {
-        // rows beyond 2m are zero in LDS, so tiles past the last block row
-        // add zeros and need no branch here
-        double blk[8];
+        if (2 * ks < cn) {
+          // rows beyond 2m are zero in LDS, so tiles past the last block row
+          // add zeros and need no branch here
+          double blk[8];
 #pragma unroll
-        for (int b = 0; b < 8; b++)
-          if (xc_needs(G, H, b)) blk[b] = base[ks * 4 * XC_RS + 16 * b];
+          for (int b = 0; b < 8; b++)
+            if (xc_needs(G, H, b)) blk[b] = base[ks * 4 * XC_RS + 16 * b];
 #pragma unroll
-        for (int q = 0; q < xc_nt(H); q++) {
-          const int TJ = xc_tj(G, xc_i0(H) + q), TK = xc_tk(G, xc_i0(H) + q);
-          acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(blk[TJ], blk[TK], acc[t][q], 0, 0, 0);
+          for (int q = 0; q < xc_nt(H); q++) {
+            const int TJ = xc_tj(G, xc_i0(H) + q), TK = xc_tk(G, xc_i0(H) + q);
+            acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(blk[TJ], blk[TK], acc[t][q], 0, 0, 0);
+          }
+#pragma unroll
+          for (int q = 0; q < xc_nt(H); q++) xc_pin(acc[t][q]);
         }
-#pragma unroll
-        for (int q = 0; q < xc_nt(H); q++) xc_pin(acc[t][q]);
         if (nt >= 0 && (ks == 1 || ks == 5))
           xc_entry(s, Qd, xs, Vn, ntb, tid + (ks == 5 ? XC_THREADS : 0), nc0, ncn, m, p0);
         // during the last chunk, F of this (walker, spin) is fetched into

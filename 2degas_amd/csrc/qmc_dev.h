@@ -55,6 +55,10 @@ struct DevSys {
   int nkxv, nkyv, kchunk;
   const double* kxv;
   const double* kyv;
+  // kprog: kxv and kyv are ascending arithmetic progressions with steps dkx,
+  // dky (lattice k-vectors), so runs of phases follow from one by products
+  int kprog;
+  double dkx, dky;
   const int* kxy;          // [nkslot] factorized rho_k slots: kxi | kyi << 8 | (k + 1) << 16
   int nkslot;
   int nke[3];              // nk_ewald of the v2 table of each type pair

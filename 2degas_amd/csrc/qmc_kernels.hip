@@ -20,6 +20,7 @@
 
 // k-vectors per thread and pass in k_pairs
 #define KMAXPT 4
+#define KRUN 8  // phases of one particle built from one sin/cos pair
 
 // ---------------------------------------------------------------------------
 // k_pairs
@@ -255,13 +256,40 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
         for (int c0 = 0; c0 < N; c0 += s.kchunk) {
           const int cn = min(s.kchunk, N - c0);
           __syncthreads();
-          for (int e = threadIdx.x; e < cn * nv; e += QMC_BLOCK) {
-            const int pc = e / nv, v = e - pc * nv;
-            const int i = c0 + pc;
-            const double arg = v < s.nkxv ? s.kxv[v] * xs[2 * i] : s.kyv[v - s.nkxv] * xs[2 * i + 1];
-            double sv, cv;
-            qsincos(arg, &sv, &cv);
-            ph[e] = make_double2(cv, sv);
+          if (s.kprog) {
+            // runs of KRUN phases: the first directly, the rest by products
+            // with e^{i dk x} (at most KRUN - 1 roundings from a direct value)
+            const int gx = (s.nkxv + KRUN - 1) / KRUN, ng = gx + (s.nkyv + KRUN - 1) / KRUN;
+            for (int e = threadIdx.x; e < cn * ng; e += QMC_BLOCK) {
+              const int pc = e / ng, g = e - pc * ng;
+              const int i = c0 + pc;
+              const bool isx = g < gx;
+              const int v0 = (isx ? g : g - gx) * KRUN, nrun = min(KRUN, (isx ? s.nkxv : s.nkyv) - v0);
+              const double u = isx ? xs[2 * i] : xs[2 * i + 1];
+              double sa, ca, sd, cd;
+              qsincos((isx ? s.kxv[v0] : s.kyv[v0]) * u, &sa, &ca);
+              qsincos((isx ? s.dkx : s.dky) * u, &sd, &cd);
+              double2* dst = ph + pc * nv + (isx ? 0 : s.nkxv) + v0;
+              dst[0] = make_double2(ca, sa);
+#pragma unroll
+              for (int r = 1; r < KRUN; r++) {
+                if (r < nrun) {
+                  const double c2 = fma(ca, cd, -sa * sd);
+                  sa = fma(ca, sd, sa * cd);
+                  ca = c2;
+                  dst[r] = make_double2(ca, sa);
+                }
+              }
+            }
+          } else {
+            for (int e = threadIdx.x; e < cn * nv; e += QMC_BLOCK) {
+              const int pc = e / nv, v = e - pc * nv;
+              const int i = c0 + pc;
+              const double arg = v < s.nkxv ? s.kxv[v] * xs[2 * i] : s.kyv[v - s.nkxv] * xs[2 * i + 1];
+              double sv, cv;
+              qsincos(arg, &sv, &cv);
+              ph[e] = make_double2(cv, sv);
+            }
           }
           __syncthreads();
           // particles of type 0 come first: [0, split) of the chunk; the

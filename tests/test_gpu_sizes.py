@@ -38,3 +38,35 @@ def test_sizes_match_oracle(nup, ndown, tmp_path, oracle_mod):
     errs = compare_eval(out, gold, ns, RTOL, f"{nup}+{ndown}")
     print("   max err", max(errs.values()), flush=True)
     assert np.array_equal(out["status"], ref["status"])
+
+
+def test_phases_off_lattice_match_oracle(tmp_path, oracle_mod):
+    """k components that are few but not a progression: the factorized rho_k
+    keeps one sin/cos per phase instead of runs of products.  Each distinct
+    kx and ky value is moved by its own relative amount, so both the oracle
+    and the library see the same off-lattice k-vectors."""
+    print("\n[gpu] off-lattice k components 37+29", flush=True)
+    dg, deck, hip = pkg("deckgen"), pkg("deck"), pkg("hip")
+    path = os.path.join(str(tmp_path), "deck")
+    dg.make_deck(path, "kp", 37, 29, 5.0, 40)
+    d = deck.read_deck(path)
+    kv = np.array(d.kvec, dtype=np.float64).reshape(-1, 2)
+    for c in range(2):
+        vals = np.unique(kv[:, c])
+        shift = {v: 1.0 + 1e-3 * np.sin(7.0 * i + c) for i, v in enumerate(vals)}
+        kv[:, c] = [v * shift[v] for v in kv[:, c]]
+    d.kvec = kv.reshape(np.shape(d.kvec))
+    x = dg.lattice_configs(8, d.nptot, d.el[0], 321)
+    o = oracle_mod.Oracle(d, twist=(1, 1))
+    ref = o.eval(x)
+    ns = o.info["n_scalar_props"]
+    gold = dict(x=x, scalars=ref["props"][:, :ns], g=ref["g"], h=ref["h"][:, :d.ntypes],
+                grad2_ph=ref["grad2_ph"], rhok=ref["props"][:, ns:])
+    ctx = hip.Context(d, twist=(1, 1))
+    try:
+        out = ctx.eval(x)
+    finally:
+        ctx.close()
+    errs = compare_eval(out, gold, ns, RTOL, "off-lattice k")
+    print("   max err", max(errs.values()), flush=True)
+    assert np.array_equal(out["status"], ref["status"])
