@@ -6,25 +6,44 @@
 // determinant kernel (k_gjm stores them beside the determinant scalars);
 // this kernel adds the backflow Jacobian, regenerated from the eta table,
 // over the pairs.
+//
+// One workgroup per (walker, spin sp); every pair with a particle of sp is
+// evaluated once, on the round-robin schedule of k_pairs: steps d = 1..32 of
+// the spin's own block (both ends are sp particles: each takes the pair's
+// part of its backflow block Q, b and of its drift and Laplacian terms
+// against the other's orbital factors) and d = 0..63 across to the other
+// type (the sp particle takes Q and b, the other one its drift and
+// Laplacian terms).  A lane keeps its own particle's sums in registers for
+// the wavefront's whole run; the partner's go to the wavefront's partial sums
+// in LDS, which are added in a fixed order at the end.
 #include "qmc_dev.h"
 
-// LDS: xs[2N], dl[4m] (F_x(j,j), F_y(j,j)), dd[3m] (diag of F_x F_x, F_x F_y, F_y F_y).
+// per-particle sums: the drift and Laplacian terms of every particle and the
+// backflow block (Q11, Q12, Q22, bx, by) of the spin's own ones
+#define FG_NZ 6
+#define FG_NQ 5
+
+// LDS: xs[2N], dl[4 * 64] (F_x(j,j), F_y(j,j)), dd[3 * 64] (diag of F_x F_x,
+// F_x F_y, F_y F_y), then four wavefront copies of z[FG_NZ][N] and qb[FG_NQ][64].
 __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
                                                      const double* __restrict__ x,
-                                                     double* __restrict__ ws, int tpp_log2) {
+                                                     double* __restrict__ ws) {
   extern __shared__ double sm[];
   __shared__ double red[QMC_BLOCK / 64];
   const int w = blockIdx.x, sp = blockIdx.y;
   const int m = pick2(s.np, sp);
   if (m == 0) return;
   const int N = s.nptot, p0 = pick2(s.ipf, sp);
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, l = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   double* wsw = ws + (int64_t)w * L.per_walker;
   double* dsc = wsw + pick2(L.off_det, sp);
   if (dsc[DS_SING] != 0.0) return;
   double* xs = sm;
   double* dl = sm + 2 * N;
-  double* dd = dl + 4 * m;
+  double* dd = dl + 4 * 64;
+  const int pstride = FG_NZ * N + FG_NQ * 64;
+  double* parts = dd + 3 * 64;
   const double* xw = x + (int64_t)w * 2 * N;
   for (int k = tid; k < 2 * N; k += QMC_BLOCK) xs[k] = xw[k];
   const double* Fd = dsc + DS_N + 2 * N + 3 * m;
@@ -34,107 +53,133 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
 #pragma unroll
     for (int c = 0; c < 3; c++) dd[3 * e + c] = Fd[7 * e + 4 + c];
   }
+  for (int k = tid; k < 4 * pstride; k += QMC_BLOCK) parts[k] = 0.0;
   __syncthreads();
 
-  const int tpp = 1 << tpp_log2;
-  const int lane = tid & (tpp - 1), grp = tid >> tpp_log2, ngrp = QMC_BLOCK >> tpp_log2;
+  const int ot = 1 - sp;
+  const int mo = s.ntypes > 1 ? pick2(s.np, ot) : 0;
+  const bool on_sp = sp ? pick2(s.iub_flag[1], sp) : pick2(s.iub_flag[0], sp);
+  const bool on_ot = mo > 0 && (sp ? pick2(s.iub_flag[1], ot) : pick2(s.iub_flag[0], ot));
+  const TabRef tA = tabref<TK_B>(s, tpair(sp, sp));
+  const TabRef tB = tabref<TK_B>(s, tpair(sp, ot));
+  // the spin's own block only feeds its own particles; the block across,
+  // their Q and b and the other type's drift and Laplacian terms
+  const int nA = (on_sp && tA.row0 >= 0) ? 32 : 0;
+  const int nB = (mo > 0 && tB.row0 >= 0 && (on_sp || on_ot)) ? 64 : 0;
+  const int nU = nA + nB;
+  const int u0 = wv * nU / 4, u1 = (wv + 1) * nU / 4;
+  double* part = parts + wv * pstride;
+  // lane l's particle of the spin: its position and orbital factors
+  const bool li = l < m;
+  const int ip = p0 + l;
+  const double xi = li ? xs[2 * ip] : 0.0, yi = li ? xs[2 * ip + 1] : 0.0;
+  double dlr[4] = {0, 0, 0, 0}, ddr[3] = {0, 0, 0};
+  if (li) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) dlr[c] = dl[4 * l + c];
+#pragma unroll
+    for (int c = 0; c < 3; c++) ddr[c] = dd[3 * l + c];
+  }
+  double iz[FG_NZ] = {0, 0, 0, 0, 0, 0}, iq[FG_NQ] = {0, 0, 0, 0, 0};
+  for (int u = u0; u < u1; u++) {
+    const bool blkA = u < nA;
+    const int d = blkA ? u + 1 : u - nA;
+    const int jl = (l + d) & 63;
+    const bool ok = li && (blkA ? (jl < m && (d < 32 || l < 32)) : jl < mo);
+    if (!ok) continue;
+    const int jp = blkA ? p0 + jl : pick2(s.ipf, ot) + jl;
+    // r = x_i - x_j; the pair's Jacobian block mb (even in r), cc r (odd)
+    const PairGeo p = pair_geo(s, xi, yi, xs[2 * jp], xs[2 * jp + 1]);
+    double eta, deta, ddeta;
+    spline3(s, blkA ? tA : tB, p.ind, p.rem, eta, deta, ddeta);
+    const double cb = deta * p.byr;
+    const double mb11 = eta + cb * p.ux * p.ux, mb12 = cb * p.ux * p.uy, mb22 = eta + cb * p.uy * p.uy;
+    const double cc = ddeta + 3.0 * cb;
+    const double a11 = mb11 * mb11 + mb12 * mb12, a12 = mb12 * (mb11 + mb22), a22 = mb12 * mb12 + mb22 * mb22;
+    // i's backflow block
+    iq[0] += mb11;
+    iq[1] += mb12;
+    iq[2] += mb22;
+    iq[3] += cc * p.ux;
+    iq[4] += cc * p.uy;
+    // j's drift and Laplacian terms against i's orbital factors (r_ji = -r)
+    double* pz = part + jp;
+    pz[0 * N] -= dlr[0] * mb11 + dlr[2] * mb12;
+    pz[1 * N] -= dlr[1] * mb11 + dlr[3] * mb12;
+    pz[2 * N] -= dlr[0] * mb12 + dlr[2] * mb22;
+    pz[3 * N] -= dlr[1] * mb12 + dlr[3] * mb22;
+    pz[4 * N] += cc * (dlr[0] * p.ux + dlr[2] * p.uy);
+    pz[5 * N] += ddr[0] * a11 + 2.0 * ddr[1] * a12 + ddr[2] * a22;
+    if (blkA) {
+      // j is a particle of the spin too: its backflow block, and i's terms
+      // against j's orbital factors
+      double* pq = part + FG_NZ * N + jl;
+      pq[0 * 64] += mb11;
+      pq[1 * 64] += mb12;
+      pq[2 * 64] += mb22;
+      pq[3 * 64] -= cc * p.ux;
+      pq[4 * 64] -= cc * p.uy;
+      const double dxr = dl[4 * jl], dxi = dl[4 * jl + 1], dyr = dl[4 * jl + 2], dyi = dl[4 * jl + 3];
+      iz[0] -= dxr * mb11 + dyr * mb12;
+      iz[1] -= dxi * mb11 + dyi * mb12;
+      iz[2] -= dxr * mb12 + dyr * mb22;
+      iz[3] -= dxi * mb12 + dyi * mb22;
+      iz[4] -= cc * (dxr * p.ux + dyr * p.uy);
+      iz[5] += dd[3 * jl] * a11 + 2.0 * dd[3 * jl + 1] * a12 + dd[3 * jl + 2] * a22;
+    }
+  }
+  if (li && u1 > u0) {
+#pragma unroll
+    for (int q = 0; q < FG_NZ; q++) part[q * N + ip] += iz[q];
+#pragma unroll
+    for (int q = 0; q < FG_NQ; q++) part[FG_NZ * N + q * 64 + l] += iq[q];
+  }
+  __syncthreads();
+
   double acc_g2 = 0.0, acc_h0 = 0.0, acc_h1 = 0.0;
   double* gdet = dsc + DS_N;
   double* Qd = gdet + 2 * N;
-  // particles in type-uniform wavefronts, partners visited type by type (as
-  // in k_pairs): the table row and whether i or c belongs to this spin are
-  // wavefront-uniform
-  const PairSlots ps = pair_slots(s, tpp_log2);
-  const int wgrp0 = __builtin_amdgcn_readfirstlane(tid >> 6) * ps.gsz;
-  for (int sl0 = 0; sl0 < ps.nslot; sl0 += ngrp) {
-    if (sl0 + wgrp0 >= ps.nslot) break;
-    const int ti = sl0 + wgrp0 >= ps.ns0 ? 1 : 0;
-    const int li = sl0 + grp - (ti ? ps.ns0 : 0);
-    const bool act = li < pick2(s.np, ti);
-    const int ip = pick2(s.ipf, ti) + li;
-    const bool on = sp ? pick2(s.iub_flag[1], ti) : pick2(s.iub_flag[0], ti);
+  for (int pp = tid; pp < N; pp += QMC_BLOCK) {
+    const int ti = ptype(s, pp);
     const bool insp = ti == sp;
-    double zxr = 0, zxi = 0, zyr = 0, zyi = 0, hq = 0, hd = 0;
-    double q11 = 0, q12 = 0, q22 = 0, bx = 0, by = 0;
-    if (act && on) {
-      const double xi = xs[2 * ip], yi = xs[2 * ip + 1];
-      for (int tj = 0; tj < s.ntypes; tj++) {
-        const TabRef tb = tabref<TK_B>(s, tpair(ti, tj));
-        const bool jin = tj == sp;
-        // pairs with neither particle in this spin contribute nothing
-        if (tb.row0 < 0 || !(insp || jin)) continue;
-        const int jb = pick2(s.ipf, tj), je = jb + pick2(s.np, tj);
-        for (int c = jb + lane; c < je; c += tpp) {
-          if (c == ip) continue;
-          PairGeo p = pair_geo(s, xi, yi, xs[2 * c], xs[2 * c + 1]);
-          double eta, deta, ddeta;
-          spline3(s, tb, p.ind, p.rem, eta, deta, ddeta);
-          const double cb = deta * p.byr;
-          const double mb11 = eta + cb * p.ux * p.ux, mb12 = cb * p.ux * p.uy,
-                       mb22 = eta + cb * p.uy * p.uy;
-          const double cc = ddeta + 3.0 * cb;
-          if (insp) {
-            q11 += mb11; q12 += mb12; q22 += mb22;
-            bx += cc * p.ux; by += cc * p.uy;
-          }
-          if (jin) {
-            const int jf = c - p0;
-            const double dxr = dl[4 * jf], dxi = dl[4 * jf + 1], dyr = dl[4 * jf + 2],
-                         dyi = dl[4 * jf + 3];
-            zxr -= dxr * mb11 + dyr * mb12;
-            zxi -= dxi * mb11 + dyi * mb12;
-            zyr -= dxr * mb12 + dyr * mb22;
-            zyi -= dxi * mb12 + dyi * mb22;
-            hq -= cc * (dxr * p.ux + dyr * p.uy);
-            const double a11 = mb11 * mb11 + mb12 * mb12, a12 = mb12 * (mb11 + mb22),
-                         a22 = mb12 * mb12 + mb22 * mb22;
-            hd += dd[3 * jf] * a11 + 2.0 * dd[3 * jf + 1] * a12 + dd[3 * jf + 2] * a22;
-          }
-        }
+    if (!insp && (s.ntypes < 2 || mo == 0)) continue;
+    const bool on = insp ? on_sp : on_ot;
+    if (!on) {
+      gdet[2 * pp] = 0.0;
+      gdet[2 * pp + 1] = 0.0;
+      continue;
+    }
+    double z[FG_NZ];
+#pragma unroll
+    for (int q = 0; q < FG_NZ; q++)
+      z[q] = ((parts[q * N + pp] + parts[pstride + q * N + pp]) + parts[2 * pstride + q * N + pp]) +
+             parts[3 * pstride + q * N + pp];
+    if (insp) {
+      const int jf = pp - p0;
+      double q[FG_NQ];
+#pragma unroll
+      for (int c = 0; c < FG_NQ; c++) {
+        const int o = FG_NZ * N + c * 64 + jf;
+        q[c] = ((parts[o] + parts[pstride + o]) + parts[2 * pstride + o]) + parts[3 * pstride + o];
       }
+      const double a11 = 1.0 + q[0], a12 = q[1], a22 = 1.0 + q[2];
+      const double dxr = dl[4 * jf], dxi = dl[4 * jf + 1], dyr = dl[4 * jf + 2], dyi = dl[4 * jf + 3];
+      z[0] += dxr * a11 + dyr * a12;
+      z[1] += dxi * a11 + dyi * a12;
+      z[2] += dxr * a12 + dyr * a22;
+      z[3] += dxi * a12 + dyi * a22;
+      z[4] += dxr * q[3] + dyr * q[4];
+      const double b11 = a11 * a11 + a12 * a12, b12 = a12 * (a11 + a22), b22 = a12 * a12 + a22 * a22;
+      z[5] += dd[3 * jf] * b11 + 2.0 * dd[3 * jf + 1] * b12 + dd[3 * jf + 2] * b22;
+      Qd[3 * jf] = q[0];
+      Qd[3 * jf + 1] = q[1];
+      Qd[3 * jf + 2] = q[2];
     }
-    for (int o = 1; o < tpp; o <<= 1) {
-      zxr += __shfl_xor(zxr, o, 64);
-      zxi += __shfl_xor(zxi, o, 64);
-      zyr += __shfl_xor(zyr, o, 64);
-      zyi += __shfl_xor(zyi, o, 64);
-      hq += __shfl_xor(hq, o, 64);
-      hd += __shfl_xor(hd, o, 64);
-      q11 += __shfl_xor(q11, o, 64);
-      q12 += __shfl_xor(q12, o, 64);
-      q22 += __shfl_xor(q22, o, 64);
-      bx += __shfl_xor(bx, o, 64);
-      by += __shfl_xor(by, o, 64);
-    }
-    if (act && lane == 0) {
-      if (on) {
-        if (insp) {
-          const int jf = ip - p0;
-          const double a11 = 1.0 + q11, a12 = q12, a22 = 1.0 + q22;
-          const double dxr = dl[4 * jf], dxi = dl[4 * jf + 1], dyr = dl[4 * jf + 2],
-                       dyi = dl[4 * jf + 3];
-          zxr += dxr * a11 + dyr * a12;
-          zxi += dxi * a11 + dyi * a12;
-          zyr += dxr * a12 + dyr * a22;
-          zyi += dxi * a12 + dyi * a22;
-          hq += dxr * bx + dyr * by;
-          const double b11 = a11 * a11 + a12 * a12, b12 = a12 * (a11 + a22),
-                       b22 = a12 * a12 + a22 * a22;
-          hd += dd[3 * jf] * b11 + 2.0 * dd[3 * jf + 1] * b12 + dd[3 * jf + 2] * b22;
-          Qd[3 * jf] = q11;
-          Qd[3 * jf + 1] = q12;
-          Qd[3 * jf + 2] = q22;
-        }
-        gdet[2 * ip] = -zxr;
-        gdet[2 * ip + 1] = -zyr;
-        acc_g2 += pick2(s.hbs2m, ti) * (zxi * zxi + zyi * zyi);
-        if (ti == 0) acc_h0 -= hq + hd;
-        else acc_h1 -= hq + hd;
-      } else {
-        gdet[2 * ip] = 0.0;
-        gdet[2 * ip + 1] = 0.0;
-      }
-    }
+    gdet[2 * pp] = -z[0];
+    gdet[2 * pp + 1] = -z[2];
+    acc_g2 += pick2(s.hbs2m, ti) * (z[1] * z[1] + z[3] * z[3]);
+    if (ti == 0) acc_h0 -= z[4] + z[5];
+    else acc_h1 -= z[4] + z[5];
   }
   acc_g2 = block_sum(acc_g2, red);
   acc_h0 = block_sum(acc_h0, red);
