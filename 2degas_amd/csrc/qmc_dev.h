@@ -237,23 +237,6 @@ __device__ __forceinline__ PairGeo pair_geo(const DevSys& s, double xi, double y
 
 __device__ __forceinline__ int ptype(const DevSys& s, int i) { return i >= s.ipf[1] && s.ntypes > 1 ? 1 : 0; }
 
-// Pair loops: particle i is handled by tpp threads that split its partners j.  The
-// particles of each type occupy whole wavefronts (each type's count is padded
-// to a multiple of 64 / tpp slots), and the partners are visited type by type,
-// so the type pair of every pair loop is wavefront-uniform: the table rows,
-// grid sizes and potential exponents are scalar operands, and a type pair
-// without a table of some kind skips that term with a scalar branch.
-struct PairSlots {
-  int gsz, ns0, nslot;
-};
-__device__ __forceinline__ PairSlots pair_slots(const DevSys& s, int tpp_log2) {
-  PairSlots p;
-  p.gsz = 64 >> tpp_log2;
-  p.ns0 = (s.np[0] + p.gsz - 1) / p.gsz * p.gsz;
-  p.nslot = p.ns0 + (s.ntypes > 1 ? (s.np[1] + p.gsz - 1) / p.gsz * p.gsz : 0);
-  return p;
-}
-
 // Maximum over the wavefront of a 32-bit key: DPP swaps inside quads, row
 // rotations by 4 and 8 inside each row of 16 lanes, then the four row maxima.
 // Every lane must be active.
