@@ -237,6 +237,39 @@ __device__ __forceinline__ PairGeo pair_geo(const DevSys& s, double xi, double y
 
 __device__ __forceinline__ int ptype(const DevSys& s, int i) { return i >= s.ipf[1] && s.ntypes > 1 ? 1 : 0; }
 
+// k_fgrad's schedule for one spin: nA = 32 steps of the spin's own block
+// (weight 3: both ends take drift terms) then nB = 64 steps across to the
+// other type (weight 2), split into four runs of equal weight, one per
+// wavefront.  A wavefront's LDS partial sums hold the backflow sums qb[5][64]
+// of the spin's particles, then zs[6][64] (drift and Laplacian terms of the
+// spin's particles) when its run has own-block steps and zo[6][64] (those of
+// the other type) when it has steps across; off[w] is where wavefront w's
+// sums start.
+#define FG_NZ 6
+#define FG_NQ 5
+struct FgPlan {
+  int u[5], off[5];
+  bool hasA[4], hasB[4];
+};
+__host__ __device__ inline FgPlan fg_plan(int nA, int nB) {
+  FgPlan f;
+  const int C = 3 * nA + 2 * nB;
+  for (int w = 0; w < 4; w++) {
+    const int T = (w * C) / 4;
+    const int u = T <= 3 * nA ? (T + 2) / 3 : nA + (T - 3 * nA + 1) / 2;
+    f.u[w] = u < nA + nB ? u : nA + nB;
+  }
+  f.u[4] = nA + nB;
+  f.off[0] = 0;
+  for (int w = 0; w < 4; w++) {
+    const bool any = f.u[w + 1] > f.u[w];
+    f.hasA[w] = any && f.u[w] < nA;
+    f.hasB[w] = any && f.u[w + 1] > nA;
+    f.off[w + 1] = f.off[w] + (any ? FG_NQ * 64 : 0) + (f.hasA[w] ? FG_NZ * 64 : 0) + (f.hasB[w] ? FG_NZ * 64 : 0);
+  }
+  return f;
+}
+
 // Maximum over the wavefront of a 32-bit key: DPP swaps inside quads, row
 // rotations by 4 and 8 inside each row of 16 lanes, then the four row maxima.
 // Every lane must be active.

@@ -18,13 +18,10 @@
 // in LDS, which are added in a fixed order at the end.
 #include "qmc_dev.h"
 
-// per-particle sums: the drift and Laplacian terms of every particle and the
-// backflow block (Q11, Q12, Q22, bx, by) of the spin's own ones
-#define FG_NZ 6
-#define FG_NQ 5
-
 // LDS: xs[2N], dl[4 * 64] (F_x(j,j), F_y(j,j)), dd[3 * 64] (diag of F_x F_x,
-// F_x F_y, F_y F_y), then four wavefront copies of z[FG_NZ][N] and qb[FG_NQ][64].
+// F_x F_y, F_y F_y), then the wavefronts' partial sums as fg_plan lays them
+// out (drift and Laplacian terms zxr, zxi, zyr, zyi, hq, hd; backflow block
+// Q11, Q12, Q22, bx, by).
 __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
                                                      const double* __restrict__ x,
                                                      double* __restrict__ ws) {
@@ -42,7 +39,6 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
   double* xs = sm;
   double* dl = sm + 2 * N;
   double* dd = dl + 4 * 64;
-  const int pstride = FG_NZ * N + FG_NQ * 64;
   double* parts = dd + 3 * 64;
   const double* xw = x + (int64_t)w * 2 * N;
   for (int k = tid; k < 2 * N; k += QMC_BLOCK) xs[k] = xw[k];
@@ -53,9 +49,6 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
 #pragma unroll
     for (int c = 0; c < 3; c++) dd[3 * e + c] = Fd[7 * e + 4 + c];
   }
-  for (int k = tid; k < 4 * pstride; k += QMC_BLOCK) parts[k] = 0.0;
-  __syncthreads();
-
   const int ot = 1 - sp;
   const int mo = s.ntypes > 1 ? pick2(s.np, ot) : 0;
   const bool on_sp = sp ? pick2(s.iub_flag[1], sp) : pick2(s.iub_flag[0], sp);
@@ -66,9 +59,13 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
   // their Q and b and the other type's drift and Laplacian terms
   const int nA = (on_sp && tA.row0 >= 0) ? 32 : 0;
   const int nB = (mo > 0 && tB.row0 >= 0 && (on_sp || on_ot)) ? 64 : 0;
-  const int nU = nA + nB;
-  const int u0 = wv * nU / 4, u1 = (wv + 1) * nU / 4;
-  double* part = parts + wv * pstride;
+  const FgPlan fp = fg_plan(nA, nB);
+  for (int k = tid; k < fp.off[4]; k += QMC_BLOCK) parts[k] = 0.0;
+  __syncthreads();
+  const int u0 = fp.u[wv], u1 = fp.u[wv + 1];
+  double* pq_w = parts + fp.off[wv];                            // qb[FG_NQ][64]
+  double* zs_w = pq_w + FG_NQ * 64;                              // zs[FG_NZ][64]
+  double* zo_w = zs_w + (fp.hasA[wv] ? FG_NZ * 64 : 0);          // zo[FG_NZ][64]
   // lane l's particle of the spin: its position and orbital factors
   const bool li = l < m;
   const int ip = p0 + l;
@@ -88,6 +85,7 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
     const bool ok = li && (blkA ? (jl < m && (d < 32 || l < 32)) : jl < mo);
     if (!ok) continue;
     const int jp = blkA ? p0 + jl : pick2(s.ipf, ot) + jl;
+    double* pz = (blkA ? zs_w : zo_w) + jl;
     // r = x_i - x_j; the pair's Jacobian block mb (even in r), cc r (odd)
     const PairGeo p = pair_geo(s, xi, yi, xs[2 * jp], xs[2 * jp + 1]);
     double eta, deta, ddeta;
@@ -103,17 +101,16 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
     iq[3] += cc * p.ux;
     iq[4] += cc * p.uy;
     // j's drift and Laplacian terms against i's orbital factors (r_ji = -r)
-    double* pz = part + jp;
-    pz[0 * N] -= dlr[0] * mb11 + dlr[2] * mb12;
-    pz[1 * N] -= dlr[1] * mb11 + dlr[3] * mb12;
-    pz[2 * N] -= dlr[0] * mb12 + dlr[2] * mb22;
-    pz[3 * N] -= dlr[1] * mb12 + dlr[3] * mb22;
-    pz[4 * N] += cc * (dlr[0] * p.ux + dlr[2] * p.uy);
-    pz[5 * N] += ddr[0] * a11 + 2.0 * ddr[1] * a12 + ddr[2] * a22;
+    pz[0 * 64] -= dlr[0] * mb11 + dlr[2] * mb12;
+    pz[1 * 64] -= dlr[1] * mb11 + dlr[3] * mb12;
+    pz[2 * 64] -= dlr[0] * mb12 + dlr[2] * mb22;
+    pz[3 * 64] -= dlr[1] * mb12 + dlr[3] * mb22;
+    pz[4 * 64] += cc * (dlr[0] * p.ux + dlr[2] * p.uy);
+    pz[5 * 64] += ddr[0] * a11 + 2.0 * ddr[1] * a12 + ddr[2] * a22;
     if (blkA) {
       // j is a particle of the spin too: its backflow block, and i's terms
       // against j's orbital factors
-      double* pq = part + FG_NZ * N + jl;
+      double* pq = pq_w + jl;
       pq[0 * 64] += mb11;
       pq[1 * 64] += mb12;
       pq[2 * 64] += mb22;
@@ -129,10 +126,12 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
     }
   }
   if (li && u1 > u0) {
+    if (fp.hasA[wv]) {
 #pragma unroll
-    for (int q = 0; q < FG_NZ; q++) part[q * N + ip] += iz[q];
+      for (int q = 0; q < FG_NZ; q++) zs_w[q * 64 + l] += iz[q];
+    }
 #pragma unroll
-    for (int q = 0; q < FG_NQ; q++) part[FG_NZ * N + q * 64 + l] += iq[q];
+    for (int q = 0; q < FG_NQ; q++) pq_w[q * 64 + l] += iq[q];
   }
   __syncthreads();
 
@@ -149,19 +148,24 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
       gdet[2 * pp + 1] = 0.0;
       continue;
     }
-    double z[FG_NZ];
+    // the wavefronts' partial sums, added in wavefront order
+    const int jx = insp ? pp - p0 : pp - pick2(s.ipf, ot);
+    double z[FG_NZ] = {0, 0, 0, 0, 0, 0}, q[FG_NQ] = {0, 0, 0, 0, 0};
+    for (int v = 0; v < 4; v++) {
+      if (fp.u[v + 1] == fp.u[v]) continue;
+      const double* pv = parts + fp.off[v];
+      if (insp) {
 #pragma unroll
-    for (int q = 0; q < FG_NZ; q++)
-      z[q] = ((parts[q * N + pp] + parts[pstride + q * N + pp]) + parts[2 * pstride + q * N + pp]) +
-             parts[3 * pstride + q * N + pp];
-    if (insp) {
-      const int jf = pp - p0;
-      double q[FG_NQ];
-#pragma unroll
-      for (int c = 0; c < FG_NQ; c++) {
-        const int o = FG_NZ * N + c * 64 + jf;
-        q[c] = ((parts[o] + parts[pstride + o]) + parts[2 * pstride + o]) + parts[3 * pstride + o];
+        for (int c = 0; c < FG_NQ; c++) q[c] += pv[c * 64 + jx];
       }
+      if (insp ? fp.hasA[v] : fp.hasB[v]) {
+        const double* zv = pv + FG_NQ * 64 + (insp ? 0 : (fp.hasA[v] ? FG_NZ * 64 : 0));
+#pragma unroll
+        for (int c = 0; c < FG_NZ; c++) z[c] += zv[c * 64 + jx];
+      }
+    }
+    if (insp) {
+      const int jf = jx;
       const double a11 = 1.0 + q[0], a12 = q[1], a22 = 1.0 + q[2];
       const double dxr = dl[4 * jf], dxi = dl[4 * jf + 1], dyr = dl[4 * jf + 2], dyi = dl[4 * jf + 3];
       z[0] += dxr * a11 + dyr * a12;
