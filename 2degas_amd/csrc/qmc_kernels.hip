@@ -293,28 +293,25 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
             }
           }
           __syncthreads();
-          // particles of type 0 come first: [0, split) of the chunk; the
-          // phase loads of two k-vectors are issued before their products
+          // particles of type 0 come first: [0, split) of the chunk, each
+          // type a loop of its own
           const int split = s.ntypes > 1 ? min(max(s.ipf[1] - c0, 0), cn) : cn;
-          for (int pc = 0; pc < cn; pc++) {
-            const double2* row = ph + pc * nv;
+          auto accum = [&](int pa, int pb, double (&rc)[KMAXPT], double (&rs)[KMAXPT]) {
+            for (int pc = pa; pc < pb; pc++) {
+              const double2* row = ph + pc * nv;
 #pragma unroll
-            for (int h = 0; h < KMAXPT; h += 2) {
-              // one ky phase for the item's two kx phases
-              const double2 b = row[oy[h]], a0 = row[ox[h]], a1 = row[ox[h + 1]];
-              if (pc < split) {
-                rc0[h] = fma(a0.x, b.x, fma(-a0.y, b.y, rc0[h]));
-                rs0[h] = fma(a0.x, b.y, fma(a0.y, b.x, rs0[h]));
-                rc0[h + 1] = fma(a1.x, b.x, fma(-a1.y, b.y, rc0[h + 1]));
-                rs0[h + 1] = fma(a1.x, b.y, fma(a1.y, b.x, rs0[h + 1]));
-              } else {
-                rc1[h] = fma(a0.x, b.x, fma(-a0.y, b.y, rc1[h]));
-                rs1[h] = fma(a0.x, b.y, fma(a0.y, b.x, rs1[h]));
-                rc1[h + 1] = fma(a1.x, b.x, fma(-a1.y, b.y, rc1[h + 1]));
-                rs1[h + 1] = fma(a1.x, b.y, fma(a1.y, b.x, rs1[h + 1]));
+              for (int h = 0; h < KMAXPT; h += 2) {
+                // one ky phase for the item's two kx phases
+                const double2 b = row[oy[h]], a0 = row[ox[h]], a1 = row[ox[h + 1]];
+                rc[h] = fma(a0.x, b.x, fma(-a0.y, b.y, rc[h]));
+                rs[h] = fma(a0.x, b.y, fma(a0.y, b.x, rs[h]));
+                rc[h + 1] = fma(a1.x, b.x, fma(-a1.y, b.y, rc[h + 1]));
+                rs[h + 1] = fma(a1.x, b.y, fma(a1.y, b.x, rs[h + 1]));
               }
             }
-          }
+          };
+          accum(0, split, rc0, rs0);
+          accum(split, cn, rc1, rs1);
         }
       } else {
 #pragma unroll
