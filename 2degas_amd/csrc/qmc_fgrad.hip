@@ -18,8 +18,9 @@
 // in LDS, which are added in a fixed order at the end.
 #include "qmc_dev.h"
 
-// LDS: xs[2N], dl[4 * 64] (F_x(j,j), F_y(j,j)), dd[3 * 64] (diag of F_x F_x,
-// F_x F_y, F_y F_y), then the wavefronts' partial sums as fg_plan lays them
+// LDS: xs[2N], dl[4][64] (F_x(j,j), F_y(j,j) as re, im), dd[3][64] (diag of
+// F_x F_x, F_x F_y, F_y F_y), component-major so that lanes reading the
+// factors of consecutive particles touch consecutive banks, then the wavefronts' partial sums as fg_plan lays them
 // out (drift and Laplacian terms zxr, zxi, zyr, zyi, hq, hd; backflow block
 // Q11, Q12, Q22, bx, by).
 __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
@@ -45,9 +46,9 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
   const double* Fd = dsc + DS_N + 2 * N + 3 * m;
   for (int e = tid; e < m; e += QMC_BLOCK) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) dl[4 * e + c] = Fd[7 * e + c];
+    for (int c = 0; c < 4; c++) dl[c * 64 + e] = Fd[7 * e + c];
 #pragma unroll
-    for (int c = 0; c < 3; c++) dd[3 * e + c] = Fd[7 * e + 4 + c];
+    for (int c = 0; c < 3; c++) dd[c * 64 + e] = Fd[7 * e + 4 + c];
   }
   const int ot = 1 - sp;
   const int mo = s.ntypes > 1 ? pick2(s.np, ot) : 0;
@@ -73,9 +74,9 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
   double dlr[4] = {0, 0, 0, 0}, ddr[3] = {0, 0, 0};
   if (li) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) dlr[c] = dl[4 * l + c];
+    for (int c = 0; c < 4; c++) dlr[c] = dl[c * 64 + l];
 #pragma unroll
-    for (int c = 0; c < 3; c++) ddr[c] = dd[3 * l + c];
+    for (int c = 0; c < 3; c++) ddr[c] = dd[c * 64 + l];
   }
   double iz[FG_NZ] = {0, 0, 0, 0, 0, 0}, iq[FG_NQ] = {0, 0, 0, 0, 0};
   for (int u = u0; u < u1; u++) {
@@ -116,13 +117,13 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
       pq[2 * 64] += mb22;
       pq[3 * 64] -= cc * p.ux;
       pq[4 * 64] -= cc * p.uy;
-      const double dxr = dl[4 * jl], dxi = dl[4 * jl + 1], dyr = dl[4 * jl + 2], dyi = dl[4 * jl + 3];
+      const double dxr = dl[jl], dxi = dl[64 + jl], dyr = dl[128 + jl], dyi = dl[192 + jl];
       iz[0] -= dxr * mb11 + dyr * mb12;
       iz[1] -= dxi * mb11 + dyi * mb12;
       iz[2] -= dxr * mb12 + dyr * mb22;
       iz[3] -= dxi * mb12 + dyi * mb22;
       iz[4] -= cc * (dxr * p.ux + dyr * p.uy);
-      iz[5] += dd[3 * jl] * a11 + 2.0 * dd[3 * jl + 1] * a12 + dd[3 * jl + 2] * a22;
+      iz[5] += dd[jl] * a11 + 2.0 * dd[64 + jl] * a12 + dd[128 + jl] * a22;
     }
   }
   if (li && u1 > u0) {
@@ -167,14 +168,14 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
     if (insp) {
       const int jf = jx;
       const double a11 = 1.0 + q[0], a12 = q[1], a22 = 1.0 + q[2];
-      const double dxr = dl[4 * jf], dxi = dl[4 * jf + 1], dyr = dl[4 * jf + 2], dyi = dl[4 * jf + 3];
+      const double dxr = dl[jf], dxi = dl[64 + jf], dyr = dl[128 + jf], dyi = dl[192 + jf];
       z[0] += dxr * a11 + dyr * a12;
       z[1] += dxi * a11 + dyi * a12;
       z[2] += dxr * a12 + dyr * a22;
       z[3] += dxi * a12 + dyi * a22;
       z[4] += dxr * q[3] + dyr * q[4];
       const double b11 = a11 * a11 + a12 * a12, b12 = a12 * (a11 + a22), b22 = a12 * a12 + a22 * a22;
-      z[5] += dd[3 * jf] * b11 + 2.0 * dd[3 * jf + 1] * b12 + dd[3 * jf + 2] * b22;
+      z[5] += dd[jf] * b11 + 2.0 * dd[64 + jf] * b12 + dd[128 + jf] * b22;
       Qd[3 * jf] = q[0];
       Qd[3 * jf + 1] = q[1];
       Qd[3 * jf + 2] = q[2];

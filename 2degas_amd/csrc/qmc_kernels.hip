@@ -21,7 +21,7 @@
 
 // k-vectors per thread and pass in k_pairs
 #define KMAXPT 4
-#define KRUN 8  // phases of one particle built from one sin/cos pair
+#define KRUN 9  // phases of one particle built from one sin/cos pair (odd: lanes' runs start in different LDS banks)
 
 // ---------------------------------------------------------------------------
 // k_pairs
