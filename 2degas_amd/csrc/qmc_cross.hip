@@ -107,10 +107,10 @@ __device__ __forceinline__ void xc_entry(const DevSys& s, const double* __restri
       v22 = -(eta + cb * pg.uy * pg.uy);
     }
   }
-  Vt[(2 * pc) * XC_RS + 2 * j] = v11;
-  Vt[(2 * pc + 1) * XC_RS + 2 * j] = v12;
-  Vt[(2 * pc) * XC_RS + 2 * j + 1] = v12;
-  Vt[(2 * pc + 1) * XC_RS + 2 * j + 1] = v22;
+  // rows 2j, 2j+1 of each of the two columns: one 16-byte store each
+  // (XC_RS is even, so both are 16-byte aligned)
+  *reinterpret_cast<double2*>(Vt + (2 * pc) * XC_RS + 2 * j) = make_double2(v11, v12);
+  *reinterpret_cast<double2*>(Vt + (2 * pc + 1) * XC_RS + 2 * j) = make_double2(v12, v22);
 }
 
 // Destination of the L2 prefetch of F: one wavefront's 64 x 16 bytes, never read.
@@ -215,7 +215,7 @@ __device__ __forceinline__ void xc_body(const DevSys& s, double* __restrict__ ds
 __global__ __launch_bounds__(XC_THREADS, 4) void k_cross(DevSys s, WsLayout L, const double* __restrict__ x,
                                                          double* __restrict__ ws,
                                                          const double* __restrict__ Fbuf) {
-  extern __shared__ double sm[];
+  extern __shared__ __attribute__((aligned(16))) double sm[];
   __shared__ double red[XC_THREADS / 64];
   // walkers and spins in the reverse of the order in which k_gjm wrote their
   // F, so the first workgroups find theirs still in the memory-side cache
