@@ -243,31 +243,30 @@ __device__ __forceinline__ int ptype(const DevSys& s, int i) { return i >= s.ipf
 // wavefront.  A wavefront's LDS partial sums hold the backflow sums qb[5][64]
 // of the spin's particles, then zs[6][64] (drift and Laplacian terms of the
 // spin's particles) when its run has own-block steps and zo[6][64] (those of
-// the other type) when it has steps across; off[w] is where wavefront w's
-// sums start.
+// the other type) when it has steps across.  Scalar functions of (nA, nB, w)
+// rather than a table, which a varying w would index in scratch memory.
 #define FG_NZ 6
 #define FG_NQ 5
-struct FgPlan {
-  int u[5], off[5];
-  bool hasA[4], hasB[4];
-};
-__host__ __device__ inline FgPlan fg_plan(int nA, int nB) {
-  FgPlan f;
-  const int C = 3 * nA + 2 * nB;
-  for (int w = 0; w < 4; w++) {
-    const int T = (w * C) / 4;
-    const int u = T <= 3 * nA ? (T + 2) / 3 : nA + (T - 3 * nA + 1) / 2;
-    f.u[w] = u < nA + nB ? u : nA + nB;
-  }
-  f.u[4] = nA + nB;
-  f.off[0] = 0;
-  for (int w = 0; w < 4; w++) {
-    const bool any = f.u[w + 1] > f.u[w];
-    f.hasA[w] = any && f.u[w] < nA;
-    f.hasB[w] = any && f.u[w + 1] > nA;
-    f.off[w + 1] = f.off[w] + (any ? FG_NQ * 64 : 0) + (f.hasA[w] ? FG_NZ * 64 : 0) + (f.hasB[w] ? FG_NZ * 64 : 0);
-  }
-  return f;
+// first step of wavefront w's run (w = 4: the end)
+__host__ __device__ inline int fg_u(int nA, int nB, int w) {
+  if (w >= 4) return nA + nB;
+  const int C = 3 * nA + 2 * nB, T = (w * C) / 4;
+  const int u = T <= 3 * nA ? (T + 2) / 3 : nA + (T - 3 * nA + 1) / 2;
+  return u < nA + nB ? u : nA + nB;
+}
+__host__ __device__ inline bool fg_hasA(int nA, int nB, int w) {
+  return fg_u(nA, nB, w + 1) > fg_u(nA, nB, w) && fg_u(nA, nB, w) < nA;
+}
+__host__ __device__ inline bool fg_hasB(int nA, int nB, int w) {
+  return fg_u(nA, nB, w + 1) > fg_u(nA, nB, w) && fg_u(nA, nB, w + 1) > nA;
+}
+// start of wavefront w's partial sums (w = 4: their total), in doubles
+__host__ __device__ inline int fg_off(int nA, int nB, int w) {
+  int o = 0;
+  for (int v = 0; v < w; v++)
+    o += (fg_u(nA, nB, v + 1) > fg_u(nA, nB, v) ? FG_NQ * 64 : 0) + (fg_hasA(nA, nB, v) ? FG_NZ * 64 : 0) +
+         (fg_hasB(nA, nB, v) ? FG_NZ * 64 : 0);
+  return o;
 }
 
 // Maximum over the wavefront of a 32-bit key: DPP swaps inside quads, row

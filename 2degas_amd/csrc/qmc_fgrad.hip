@@ -60,13 +60,13 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
   // their Q and b and the other type's drift and Laplacian terms
   const int nA = (on_sp && tA.row0 >= 0) ? 32 : 0;
   const int nB = (mo > 0 && tB.row0 >= 0 && (on_sp || on_ot)) ? 64 : 0;
-  const FgPlan fp = fg_plan(nA, nB);
-  for (int k = tid; k < fp.off[4]; k += QMC_BLOCK) parts[k] = 0.0;
+  for (int k = tid; k < fg_off(nA, nB, 4); k += QMC_BLOCK) parts[k] = 0.0;
   __syncthreads();
-  const int u0 = fp.u[wv], u1 = fp.u[wv + 1];
-  double* pq_w = parts + fp.off[wv];                            // qb[FG_NQ][64]
+  const int u0 = fg_u(nA, nB, wv), u1 = fg_u(nA, nB, wv + 1);
+  const bool hasA = fg_hasA(nA, nB, wv);
+  double* pq_w = parts + fg_off(nA, nB, wv);                     // qb[FG_NQ][64]
   double* zs_w = pq_w + FG_NQ * 64;                              // zs[FG_NZ][64]
-  double* zo_w = zs_w + (fp.hasA[wv] ? FG_NZ * 64 : 0);          // zo[FG_NZ][64]
+  double* zo_w = zs_w + (hasA ? FG_NZ * 64 : 0);                 // zo[FG_NZ][64]
   // lane l's particle of the spin: its position and orbital factors
   const bool li = l < m;
   const int ip = p0 + l;
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
     }
   }
   if (li && u1 > u0) {
-    if (fp.hasA[wv]) {
+    if (hasA) {
 #pragma unroll
       for (int q = 0; q < FG_NZ; q++) zs_w[q * 64 + l] += iz[q];
     }
@@ -152,15 +152,17 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
     // the wavefronts' partial sums, added in wavefront order
     const int jx = insp ? pp - p0 : pp - pick2(s.ipf, ot);
     double z[FG_NZ] = {0, 0, 0, 0, 0, 0}, q[FG_NQ] = {0, 0, 0, 0, 0};
+#pragma unroll
     for (int v = 0; v < 4; v++) {
-      if (fp.u[v + 1] == fp.u[v]) continue;
-      const double* pv = parts + fp.off[v];
+      if (fg_u(nA, nB, v + 1) == fg_u(nA, nB, v)) continue;
+      const double* pv = parts + fg_off(nA, nB, v);
+      const bool hA = fg_hasA(nA, nB, v);
       if (insp) {
 #pragma unroll
         for (int c = 0; c < FG_NQ; c++) q[c] += pv[c * 64 + jx];
       }
-      if (insp ? fp.hasA[v] : fp.hasB[v]) {
-        const double* zv = pv + FG_NQ * 64 + (insp ? 0 : (fp.hasA[v] ? FG_NZ * 64 : 0));
+      if (insp ? hA : fg_hasB(nA, nB, v)) {
+        const double* zv = pv + FG_NQ * 64 + (insp ? 0 : (hA ? FG_NZ * 64 : 0));
 #pragma unroll
         for (int c = 0; c < FG_NZ; c++) z[c] += zv[c * 64 + jx];
       }
