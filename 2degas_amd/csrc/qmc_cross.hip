@@ -98,7 +98,8 @@ __device__ __forceinline__ void xc_entry(const DevSys& s, const double* __restri
       v12 = Qd[3 * j + 1];
       v22 = 1.0 + Qd[3 * j + 2];
     } else if (tb.row0 >= 0) {
-      PairGeo pg = pair_geo(s, xs[2 * jp], xs[2 * jp + 1], xs[2 * ip], xs[2 * ip + 1]);
+      const double2 xj2 = ld2(xs, jp), xi2 = ld2(xs, ip);
+      PairGeo pg = pair_geo(s, xj2.x, xj2.y, xi2.x, xi2.y);
       double eta, deta, ddeta;
       spline3(s, tb, pg.ind, pg.rem, eta, deta, ddeta);
       const double cb = deta * pg.byr;

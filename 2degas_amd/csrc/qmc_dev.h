@@ -221,6 +221,11 @@ struct PairGeo {
 // operations and a bit insert where round() takes six.
 __device__ __forceinline__ double qround(double t) { return trunc(t + copysign(0.49999999999999994, t)); }
 
+// Entry i of an LDS array of (x, y) pairs as one 16-byte read (the array
+// starts 16-byte aligned): two 8-byte reads of neighbouring lanes' pairs
+// would each stride the banks at 16 bytes.
+__device__ __forceinline__ double2 ld2(const double* a, int i) { return *reinterpret_cast<const double2*>(a + 2 * i); }
+
 __device__ __forceinline__ PairGeo pair_geo(const DevSys& s, double xi, double yi, double xj,
                                             double yj) {
   PairGeo p;

@@ -26,7 +26,7 @@
 __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
                                                      const double* __restrict__ x,
                                                      double* __restrict__ ws) {
-  extern __shared__ double sm[];
+  extern __shared__ __attribute__((aligned(16))) double sm[];
   __shared__ double red[QMC_BLOCK / 64];
   const int w = blockIdx.x, sp = blockIdx.y;
   const int m = pick2(s.np, sp);
@@ -88,7 +88,8 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
     const int jp = blkA ? p0 + jl : pick2(s.ipf, ot) + jl;
     double* pz = (blkA ? zs_w : zo_w) + jl;
     // r = x_i - x_j; the pair's Jacobian block mb (even in r), cc r (odd)
-    const PairGeo p = pair_geo(s, xi, yi, xs[2 * jp], xs[2 * jp + 1]);
+    const double2 xj2 = ld2(xs, jp);
+    const PairGeo p = pair_geo(s, xi, yi, xj2.x, xj2.y);
     double eta, deta, ddeta;
     spline3(s, blkA ? tA : tB, p.ind, p.rem, eta, deta, ddeta);
     const double cb = deta * p.byr;

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
                                                      const double* __restrict__ x,
                                                      double* __restrict__ ws,
                                                      double* __restrict__ props) {
-  extern __shared__ double sm[];
+  extern __shared__ __attribute__((aligned(16))) double sm[];
   const int w = blockIdx.x;
   const int N = s.nptot;
   double* xs = sm;
@@ -102,7 +102,8 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
         const TpInfo* T = s.tpi + tpair(ti, tj);
         const TabRef tu2{T->row[TK_U2], T->ng[TK_U2]}, tu3{T->row[TK_U3], T->ng[TK_U3]},
                      tb{T->row[TK_B], T->ng[TK_B]}, tv{T->row[TK_V], T->ng[TK_V]};
-        PairGeo p = pair_geo(s, xs[2 * i], xs[2 * i + 1], xs[2 * j], xs[2 * j + 1]);
+        const double2 xi2 = ld2(xs, i), xj2 = ld2(xs, j);
+        PairGeo p = pair_geo(s, xi2.x, xi2.y, xj2.x, xj2.y);
         double ja[PP_NQ] = {0, 0, 0, 0, 0, 0};
         double hh = 0.0;
         double f, df, ddf;
@@ -191,12 +192,14 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
       const int jl = (l + d) & 63;
       if (tu3.row0 >= 0 && l < pick2(s.np, ti) && jl < pick2(s.np, tj) && (ti != tj || d < 32 || l < 32)) {
         const int i = pick2(s.ipf, ti) + l, j = pick2(s.ipf, tj) + jl;
-        PairGeo p = pair_geo(s, xs[2 * i], xs[2 * i + 1], xs[2 * j], xs[2 * j + 1]);
+        const double2 xi2 = ld2(xs, i), xj2 = ld2(xs, j);
+        PairGeo p = pair_geo(s, xi2.x, xi2.y, xj2.x, xj2.y);
         double t, dt, ddt;
         spline3(s, tu3, p.ind, p.rem, t, dt, ddt);
         const double c = dt * p.byr;
         const double m11 = t + c * p.ux * p.ux, m12 = c * p.ux * p.uy, m22 = t + c * p.uy * p.uy;
-        const double dGx = Gs[2 * i] - Gs[2 * j], dGy = Gs[2 * i + 1] - Gs[2 * j + 1];
+        const double2 Gi = ld2(Gs, i), Gj = ld2(Gs, j);
+        const double dGx = Gi.x - Gj.x, dGy = Gi.y - Gj.y;
         const double g3x = m11 * dGx + m12 * dGy, g3y = m12 * dGx + m22 * dGy;
         const double nm2 = m11 * m11 + 2.0 * m12 * m12 + m22 * m22;
         const double h3 = (ddt + 3.0 * c) * (p.ux * dGx + p.uy * dGy) - nm2;
@@ -414,7 +417,7 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_det(DevSys s, WsLayout L,
                                                    double* __restrict__ Fbuf,
                                                    double* __restrict__ Abuf, int kc,
                                                    int tpp_log2) {
-  extern __shared__ double sm[];
+  extern __shared__ __attribute__((aligned(16))) double sm[];
   __shared__ int s_piv[QMC_MAX_ORB];
   __shared__ int s_l, s_sing;
   __shared__ double s_pmax;
