@@ -106,42 +106,58 @@ __device__ __forceinline__ zd zrcp(zd a) {
   return zd{a.x / d, -a.y / d};
 }
 
+// A constant of qsincos; with SREG it passes through an empty asm
+// statement bound to a scalar register pair, which keeps it in SGPRs and
+// stops it being hoisted out of a loop into vector registers (the same value
+// either way).
+template <bool SREG>
+__device__ __forceinline__ double qk(double c) {
+  if (SREG) __asm__ volatile("" : "+s"(c));
+  return c;
+}
+
 // sin and cos of x for |x| < 2^30 (absolute error below 1.4e-16).  Cody-Waite
 // reduction r = x - n pi/2 with pi/2 split over three doubles (the first
 // product is exact inside the FMA), then the Taylor series of sin r and cos r
 // on |r| <= pi/4 through r^17 and r^18 (coefficients 1/k!, truncation below
 // 1e-19), turned by the quadrant n mod 4.  Uses a handful of registers, where
 // the library routine's wide-argument path needs about a hundred.
-__device__ __forceinline__ void qsincos(double x, double* sn, double* cs) {
-  const double n = rint(x * 0x1.45f306dc9c883p-1);
-  double r = fma(-n, 0x1.921fb54442d18p+0, x);
-  r = fma(-n, 0x1.1a62633145c07p-54, r);
-  r = fma(-n, -0x1.f1976b7ed8fbcp-110, r);
+template <bool SREG>
+__device__ __forceinline__ void qsincos_t(double x, double* sn, double* cs) {
+  const double n = rint(x * qk<SREG>(0x1.45f306dc9c883p-1));
+  double r = fma(-n, qk<SREG>(0x1.921fb54442d18p+0), x);
+  r = fma(-n, qk<SREG>(0x1.1a62633145c07p-54), r);
+  r = fma(-n, qk<SREG>(-0x1.f1976b7ed8fbcp-110), r);
   const double r2 = r * r;
-  double ps = 1.0 / 355687428096000.0;
-  ps = fma(ps, r2, -1.0 / 1307674368000.0);
-  ps = fma(ps, r2, 1.0 / 6227020800.0);
-  ps = fma(ps, r2, -1.0 / 39916800.0);
-  ps = fma(ps, r2, 1.0 / 362880.0);
-  ps = fma(ps, r2, -1.0 / 5040.0);
-  ps = fma(ps, r2, 1.0 / 120.0);
-  ps = fma(ps, r2, -1.0 / 6.0);
+  double ps = qk<SREG>(1.0 / 355687428096000.0);
+  ps = fma(ps, r2, qk<SREG>(-1.0 / 1307674368000.0));
+  ps = fma(ps, r2, qk<SREG>(1.0 / 6227020800.0));
+  ps = fma(ps, r2, qk<SREG>(-1.0 / 39916800.0));
+  ps = fma(ps, r2, qk<SREG>(1.0 / 362880.0));
+  ps = fma(ps, r2, qk<SREG>(-1.0 / 5040.0));
+  ps = fma(ps, r2, qk<SREG>(1.0 / 120.0));
+  ps = fma(ps, r2, qk<SREG>(-1.0 / 6.0));
   const double sr = fma(r * r2, ps, r);
-  double pc = -1.0 / 6402373705728000.0;
-  pc = fma(pc, r2, 1.0 / 20922789888000.0);
-  pc = fma(pc, r2, -1.0 / 87178291200.0);
-  pc = fma(pc, r2, 1.0 / 479001600.0);
-  pc = fma(pc, r2, -1.0 / 3628800.0);
-  pc = fma(pc, r2, 1.0 / 40320.0);
-  pc = fma(pc, r2, -1.0 / 720.0);
-  pc = fma(pc, r2, 1.0 / 24.0);
-  pc = fma(pc, r2, -0.5);
+  double pc = qk<SREG>(-1.0 / 6402373705728000.0);
+  pc = fma(pc, r2, qk<SREG>(1.0 / 20922789888000.0));
+  pc = fma(pc, r2, qk<SREG>(-1.0 / 87178291200.0));
+  pc = fma(pc, r2, qk<SREG>(1.0 / 479001600.0));
+  pc = fma(pc, r2, qk<SREG>(-1.0 / 3628800.0));
+  pc = fma(pc, r2, qk<SREG>(1.0 / 40320.0));
+  pc = fma(pc, r2, qk<SREG>(-1.0 / 720.0));
+  pc = fma(pc, r2, qk<SREG>(1.0 / 24.0));
+  pc = fma(pc, r2, qk<SREG>(-0.5));
   const double cr = fma(pc, r2, 1.0);
   const int q = (int)((long long)n & 3);
   const double sv = (q & 1) ? cr : sr, cv = (q & 1) ? sr : cr;
   *sn = (q & 2) ? -sv : sv;
   *cs = ((q + 1) & 2) ? -cv : cv;
 }
+
+__device__ __forceinline__ void qsincos(double x, double* sn, double* cs) { qsincos_t<false>(x, sn, cs); }
+// qsincos with its coefficients kept in scalar registers, for loops whose
+// vector registers are all taken
+__device__ __forceinline__ void qsincos_sreg(double x, double* sn, double* cs) { qsincos_t<true>(x, sn, cs); }
 
 // x <- a x mod 2^48 with a = 34522712143931 (the limbs 502, 1521, 4071, 2107
 // of reptation.f:5349-5354); u = x / 2^48 exactly.  Unsigned 64-bit products

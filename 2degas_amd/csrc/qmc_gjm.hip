@@ -253,44 +253,23 @@ __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, dou
   const int j = col;
   const double qx = sh.q[2 * j], qy = sh.q[2 * j + 1];
   double* Fd = ws + (int64_t)w * L.per_walker + pick2(L.off_det, sp) + DS_N + 2 * s.nptot + 3 * m;
-  // diag(F_a F_b)_j = diag(A^-1 G_a G_b A)_j = sum_k' S(p_j, k') (-g^a g^b)_{p_k'} A(p_k', j) / piv_j,
-  // O(m^2) in all: the wavefronts of row half 0, lane (lr, lc) summing
-  // k' = lr mod 4 for particle j, then the four partial sums combined
-  if (rh == 0) {
-    zd zxx = zmk(0.0, 0.0), zxy = zmk(0.0, 0.0), zyy = zmk(0.0, 0.0);
-    if (j < m) {
-      const int prj = sh.prow[j], swj = gm_swz(prj);
-      for (int kp = lr; kp < m; kp += 4) {
-        const int o = sh.prow[kp];
-        const double gx = sh.g[2 * o], gy = sh.g[2 * o + 1];
-        double sv, cv;
-        qsincos(gx * qx + gy * qy, &sv, &cv);
-        const int ad = prj * 64 + (kp ^ swj);
-        const zd wv = zmul(zmk(sh.Mre[ad], sh.Mim[ad]), zmk(cv, sv));
-        const double cxx = -gx * gx, cxy = -gx * gy, cyy = -gy * gy;
-        zxx = zmk(fma(cxx, wv.x, zxx.x), fma(cxx, wv.y, zxx.y));
-        zxy = zmk(fma(cxy, wv.x, zxy.x), fma(cxy, wv.y, zxy.y));
-        zyy = zmk(fma(cyy, wv.x, zyy.x), fma(cyy, wv.y, zyy.y));
-      }
-    }
-#pragma unroll
-    for (int o = 16; o < 64; o <<= 1) {
-      zxx.x += __shfl_xor(zxx.x, o, 64);
-      zxx.y += __shfl_xor(zxx.y, o, 64);
-      zxy.x += __shfl_xor(zxy.x, o, 64);
-      zxy.y += __shfl_xor(zxy.y, o, 64);
-      zyy.x += __shfl_xor(zyy.x, o, 64);
-      zyy.y += __shfl_xor(zyy.y, o, 64);
-    }
-    if (j < m && lr == 0) {
-      const zd pi = sh.pinv[j];
-      Fd[7 * j + 4] = zmul(zxx, pi).x;
-      Fd[7 * j + 5] = zmul(zxy, pi).x;
-      Fd[7 * j + 6] = zmul(zyy, pi).x;
-    }
-  }
   // H_b = S G'_b for output column tile ct (particles j), row tiles 2 rh + t:
-  // h[t][b][0] real, h[t][b][1] imaginary
+  // h[t][b][0] real, h[t][b][1] imaginary.  Beside it the wavefronts of row
+  // half 0 form diag(F_a F_b)_j = diag(A^-1 G_a G_b A)_j
+  // = sum_k' S(p_j, k') (-g^a g^b)_{p_k'} A(p_k', j) / piv_j from the same
+  // A(p_k', j) values (lane (lr, lc) summing k' = lr mod 4 for particle j, the
+  // four partial sums combined after the loop), O(m^2) in all.  The sin/cos
+  // constants stay in scalar registers here, where the accumulators take the
+  // vector ones.
+  const bool dg = rh == 0;
+  int prj = 0, swj = 0;
+  zd pij = zmk(0.0, 0.0);
+  if (dg && j < m) {
+    prj = sh.prow[j];
+    swj = gm_swz(prj);
+    pij = sh.pinv[j];
+  }
+  double dxx = 0.0, dxy = 0.0, dyy = 0.0;
   gm_d4 h[2][2][2];
 #pragma unroll
   for (int t = 0; t < 2; t++)
@@ -303,11 +282,18 @@ __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, dou
       const int o = sh.prow[kp];
       const double gx = sh.g[2 * o], gy = sh.g[2 * o + 1];
       double sv, cv;
-      qsincos(gx * qx + gy * qy, &sv, &cv);
+      qsincos_sreg(gx * qx + gy * qy, &sv, &cv);
       bx_re = -gx * sv;
       bx_im = gx * cv;
       by_re = -gy * sv;
       by_im = gy * cv;
+      if (dg) {
+        const int ad = prj * 64 + (kp ^ swj);
+        const double wr = zmul(zmul(zmk(sh.Mre[ad], sh.Mim[ad]), pij), zmk(cv, sv)).x;
+        dxx = fma(-gx * gx, wr, dxx);
+        dxy = fma(-gx * gy, wr, dxy);
+        dyy = fma(-gy * gy, wr, dyy);
+      }
     }
     zd a[2];
 #pragma unroll
@@ -329,6 +315,19 @@ __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, dou
       h[t][0][1] = gm_mfma(a[t].y, bx_re, h[t][0][1]);
       h[t][1][0] = gm_mfma(-a[t].y, by_im, h[t][1][0]);
       h[t][1][1] = gm_mfma(a[t].y, by_re, h[t][1][1]);
+    }
+  }
+  if (dg) {
+#pragma unroll
+    for (int o = 16; o < 64; o <<= 1) {
+      dxx += __shfl_xor(dxx, o, 64);
+      dxy += __shfl_xor(dxy, o, 64);
+      dyy += __shfl_xor(dyy, o, 64);
+    }
+    if (j < m && lr == 0) {
+      Fd[7 * j + 4] = dxx;
+      Fd[7 * j + 5] = dxy;
+      Fd[7 * j + 6] = dyy;
     }
   }
   // F_b(k, j) = H_b(p_k, j) / piv_k
