@@ -57,7 +57,7 @@ __device__ __forceinline__ void pp_add(double* part, int N, int p, const double 
     if (q < nq) part[q * N + p] += v[q];
 }
 
-__global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
+__global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs(DevSys s, WsLayout L,
                                                      const double* __restrict__ x,
                                                      double* __restrict__ ws,
                                                      double* __restrict__ props) {
@@ -271,8 +271,8 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
               const int v0 = (isx ? g : g - gx) * KRUN, nrun = min(KRUN, (isx ? s.nkxv : s.nkyv) - v0);
               const double u = isx ? xs[2 * i] : xs[2 * i + 1];
               double sa, ca, sd, cd;
-              qsincos((isx ? s.kxv[v0] : s.kyv[v0]) * u, &sa, &ca);
-              qsincos((isx ? s.dkx : s.dky) * u, &sd, &cd);
+              qsincos_sreg((isx ? s.kxv[v0] : s.kyv[v0]) * u, &sa, &ca);
+              qsincos_sreg((isx ? s.dkx : s.dky) * u, &sd, &cd);
               double2* dst = ph + pc * nv + (isx ? 0 : s.nkxv) + v0;
               dst[0] = make_double2(ca, sa);
 #pragma unroll
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
               const int i = c0 + pc;
               const double arg = v < s.nkxv ? s.kxv[v] * xs[2 * i] : s.kyv[v - s.nkxv] * xs[2 * i + 1];
               double sv, cv;
-              qsincos(arg, &sv, &cv);
+              qsincos_sreg(arg, &sv, &cv);
               ph[e] = make_double2(cv, sv);
             }
           }
@@ -325,13 +325,13 @@ __global__ __launch_bounds__(QMC_BLOCK, 4) void k_pairs(DevSys s, WsLayout L,
           const int split = s.ntypes > 1 ? s.ipf[1] : N;
           for (int i = 0; i < split; i++) {
             double sv, cv;
-            qsincos(kx * xs[2 * i] + ky * xs[2 * i + 1], &sv, &cv);
+            qsincos_sreg(kx * xs[2 * i] + ky * xs[2 * i + 1], &sv, &cv);
             rc0[q] += cv;
             rs0[q] += sv;
           }
           for (int i = split; i < N; i++) {
             double sv, cv;
-            qsincos(kx * xs[2 * i] + ky * xs[2 * i + 1], &sv, &cv);
+            qsincos_sreg(kx * xs[2 * i] + ky * xs[2 * i + 1], &sv, &cv);
             rc1[q] += cv;
             rs1[q] += sv;
           }
