@@ -20,9 +20,9 @@
 
 // LDS: xs[2N], dl[4][64] (F_x(j,j), F_y(j,j) as re, im), dd[3][64] (diag of
 // F_x F_x, F_x F_y, F_y F_y), component-major so that lanes reading the
-// factors of consecutive particles touch consecutive banks, then the wavefronts' partial sums as fg_plan lays them
-// out (drift and Laplacian terms zxr, zxi, zyr, zyi, hq, hd; backflow block
-// Q11, Q12, Q22, bx, by).
+// factors of consecutive particles touch consecutive banks, then the
+// wavefronts' partial sums where fg_off puts them (drift and Laplacian terms
+// zxr, zxi, zyr, zyi, hq, hd; backflow block Q11, Q12, Q22, bx, by).
 __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
                                                      const double* __restrict__ x,
                                                      double* __restrict__ ws) {
