@@ -400,9 +400,11 @@ __global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs(DevSys s, WsLayout L,
     sc[SC_LTF] = acc_ltf;
     sc[SC_H0] = acc_h0;
     sc[SC_H1] = acc_h1;
-    sc[SC_P0] = s.tailc[0] + acc_v0 + pk0 + c0;
-    sc[SC_P1] = s.tailc[1] + acc_v1 + pk1 + c1;
-    sc[SC_PT] = s.tailc_tot + acc_v0 + acc_v1 + pkt + ct;
+    // the pair-potential tail constants are added in k_final (held here
+    // through the pair loops, they had taken scalar registers the loops need)
+    sc[SC_P0] = acc_v0 + pk0 + c0;
+    sc[SC_P1] = acc_v1 + pk1 + c1;
+    sc[SC_PT] = acc_v0 + acc_v1 + pkt + ct;
   }
 }
 
@@ -858,8 +860,8 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_final(DevSys s, WsLayout L, doubl
     p[s.jltf] = ltf;
 #pragma unroll
     for (int t = 0; t < 2; t++)
-      if (t < s.ntypes) p[s.jpot[t]] = sc[SC_P0 + t] / s.np[t];
-    p[s.jpot0] = (sc[SC_PT] + s.v0) / s.npnorm;
+      if (t < s.ntypes) p[s.jpot[t]] = (s.tailc[t] + sc[SC_P0 + t]) / s.np[t];
+    p[s.jpot0] = (s.tailc_tot + sc[SC_PT] + s.v0) / s.npnorm;
     p[s.jetot] = p[s.jpot0] + p[s.jkin0] + p[s.jefp];
     p[s.je2] = p[s.jetot] * p[s.jetot];
     if (status) status[w] = (guard ? 1 : 0) | (sing ? 2 : 0);
