@@ -9,7 +9,8 @@
 // then accepts or rejects the whole slide with the path weight
 // (reptation_metropolis_test, reptation.f:1865-1895).  On the device a move is
 // k_rmc_begin, then for idelta = 1..mdelta a proposal kernel, one batched
-// evaluation of every reptile and a bead kernel, then k_rmc_finish.  A
+// evaluation of every reptile and a bead kernel, then k_rmc_path and
+// k_rmc_finish.  A
 // reptile whose ndelta is below idelta carries its end bead through the
 // evaluation unchanged and draws nothing.
 #include "qmc_internal.h"
@@ -57,55 +58,87 @@ __global__ void k_rmc_begin(RmcK k, const int32_t* __restrict__ ends, int64_t* _
   mv[3 * r + 2] = idir == 1 ? ends[2 * r + 1] : ends[2 * r];
 }
 
+// a^n mod 2^48 for rannyu's multiplier: the stream n draws ahead is a^n x.
+__device__ __forceinline__ uint64_t lcg_pow(uint64_t n) {
+  uint64_t p = 1, q = 34522712143931ULL;
+  for (; n; n >>= 1) {
+    if (n & 1) p = (p * q) & ((1ULL << 48) - 1ULL);
+    q = (q * q) & ((1ULL << 48) - 1ULL);
+  }
+  return p;
+}
+
 // Bead proposal idelta (reptation.f:1929-1955): x_new = chi - var g_j + x_j
 // from bead j, with wold = -sum chi^2/var, dold = sum drift^2/var and
 // lng = -sum (chi + drift)^2/var.  sc[r] = {wold, dold, lng}.
-__global__ void k_rmc_propose(RmcK k, int idelta, const double* __restrict__ X,
-                              const double* __restrict__ G, const int32_t* __restrict__ mv,
-                              int64_t* __restrict__ rng, double* __restrict__ xn,
-                              double* __restrict__ sc, const double* __restrict__ g2,
-                              double* __restrict__ g2n) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= k.nr) return;
-  const int idir = mv[3 * r], nd = mv[3 * r + 1], js = mv[3 * r + 2];
-  const int N = k.N;
-  double* xo = xn + (int64_t)r * 2 * N;
-  g2n[r] = g2[r];
-  if (idelta > nd) {
-    const double* xe = X + ((int64_t)r * k.nbuf + js) * 2 * N;
-    for (int e = 0; e < 2 * N; e++) xo[e] = xe[e];
-    return;
+// One wavefront per reptile.  The reference draws two uniforms per moved
+// coordinate in coordinate order, so the coordinate with ordinal o among the
+// moved ones takes draws 2o+1 and 2o+2 of the reptile's stream: each lane
+// jumps there with a^(2o+1).  The per-coordinate terms go to LDS and three
+// lanes add them in the reference's order.
+__global__ __launch_bounds__(QMC_BLOCK) void k_rmc_propose(
+    RmcK k, int idelta, const double* __restrict__ X, const double* __restrict__ G,
+    const int32_t* __restrict__ mv, int64_t* __restrict__ rng, double* __restrict__ xn,
+    double* __restrict__ sc, const double* __restrict__ g2, double* __restrict__ g2n) {
+  extern __shared__ double rp_sm[];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = blockIdx.x * (QMC_BLOCK / 64) + wv;
+  // every wavefront reaches the barrier below: no early return
+  const bool live = r < k.nr;
+  const int idir = live ? mv[3 * r] : 0, nd = live ? mv[3 * r + 1] : 0, js = live ? mv[3 * r + 2] : 0;
+  const bool prop = live && idelta <= nd;
+  const int N = k.N, n2 = 2 * N;
+  double* xo = xn + (int64_t)r * n2;
+  if (live && lane == 0) g2n[r] = g2[r];
+  if (live && !prop) {
+    const double* xe = X + ((int64_t)r * k.nbuf + js) * n2;
+    for (int e = lane; e < n2; e += 64) xo[e] = xe[e];
   }
   const int j = rslot(js + idir * (idelta - 1), k.nbuf);
-  const double* xj = X + ((int64_t)r * k.nbuf + j) * 2 * N;
-  const double* gj = G + ((int64_t)r * k.nbuf + j) * 2 * N;
-  uint64_t st = (uint64_t)rng[r];
+  const double* xj = X + ((int64_t)r * k.nbuf + j) * n2;
+  const double* gj = G + ((int64_t)r * k.nbuf + j) * n2;
+  const uint64_t st0 = prop ? (uint64_t)rng[r] : 0;
   const double pi = 3.141592653589793;
-  double wold = 0.0, lng = 0.0, dold = 0.0;
-  for (int t = 0; t < k.ntypes; t++) {
-    if (!k.mov[t]) continue;
-    const double var = k.var[t];
-    double w = 0.0, wg = 0.0, d = 0.0;
-    for (int ip = k.ipf[t]; ip < k.ipf[t] + k.np[t]; ip++)
-      for (int dm = 0; dm < 2; dm++) {
-        const double drift = -var * gj[2 * ip + dm];
-        d = d + drift * drift;
-        double ua, ub;
-        if (k.bm_order == 0) { ua = rannyu(st); ub = rannyu(st); }
-        else { ub = rannyu(st); ua = rannyu(st); }
-        const double chi = cos(pi * ua) * sqrt(-2.0 * var * log(ub));
-        w = w - chi * chi;
-        xo[2 * ip + dm] = chi + drift + xj[2 * ip + dm];
-        wg = wg - (chi + drift) * (chi + drift);
-      }
-    wold = wold + w * k.vari[t];
-    dold = dold + d * k.vari[t];
-    lng = lng + wg * k.vari[t];
+  const uint64_t msk = (1ULL << 48) - 1ULL, a = 34522712143931ULL;
+  const bool two = k.ntypes > 1;
+  const int m0 = k.mov[0] ? 2 * k.np[0] : 0;             // moved coordinates of type 0
+  const int nmov = m0 + (two && k.mov[1] ? 2 * k.np[1] : 0);
+  double* tw = rp_sm + wv * 3 * n2;  // chi^2, drift^2, (chi + drift)^2 per coordinate
+  double* td = tw + n2;
+  double* tg = td + n2;
+  for (int e = lane; prop && e < n2; e += 64) {
+    const int t = two && e >= 2 * k.ipf[1] ? 1 : 0;
+    if (!(t ? k.mov[1] : k.mov[0])) continue;
+    const int o = t ? m0 + e - 2 * k.ipf[1] : e - 2 * k.ipf[0];
+    const double var = t ? k.var[1] : k.var[0];
+    const uint64_t x1 = (lcg_pow(2 * (uint64_t)o + 1) * st0) & msk, x2 = (a * x1) & msk;
+    const double u1 = (double)x1 * 0x1p-48, u2 = (double)x2 * 0x1p-48;
+    const double ua = k.bm_order == 0 ? u1 : u2, ub = k.bm_order == 0 ? u2 : u1;
+    const double drift = -var * gj[e];
+    const double chi = cos(pi * ua) * sqrt(-2.0 * var * log(ub));
+    xo[e] = chi + drift + xj[e];
+    const double cd = chi + drift;
+    tw[e] = chi * chi;
+    td[e] = drift * drift;
+    tg[e] = cd * cd;
   }
-  rng[r] = (int64_t)st;
-  sc[3 * r] = wold;
-  sc[3 * r + 1] = dold;
-  sc[3 * r + 2] = lng;
+  __syncthreads();
+  if (prop && lane < 3) {
+    const double* tq = lane == 0 ? tw : (lane == 1 ? td : tg);
+    double acc = 0.0;
+    for (int t = 0; t < k.ntypes; t++) {
+      if (!(t ? k.mov[1] : k.mov[0])) continue;
+      const int e0 = 2 * (t ? k.ipf[1] : k.ipf[0]), e1 = e0 + 2 * (t ? k.np[1] : k.np[0]);
+      double v = 0.0;
+      if (lane == 1)
+        for (int e = e0; e < e1; e++) v = v + tq[e];
+      else
+        for (int e = e0; e < e1; e++) v = v - tq[e];
+      acc = acc + v * (t ? k.vari[1] : k.vari[0]);
+    }
+    sc[3 * r + (lane == 0 ? 0 : (lane == 1 ? 1 : 2))] = acc;
+    if (lane == 0) rng[r] = (int64_t)((lcg_pow(2 * (uint64_t)nmov) * st0) & msk);
+  }
 }
 
 // Langevin test of bead idelta and the ring update (reptation.f:1956-2104).
@@ -216,29 +249,54 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_rmc_step(
   for (int e = lane; e < NP; e += 64) pq[e] = (e == k.jacc) ? p : ps[e];
 }
 
-// path_props scalars (reptation.f:1566-1577) for a window starting at kf:
-// mean over beads ntauskip+1..ntau-ntauskip, E from the two end beads.
-// Lanes own properties; each sums its beads in the reference's order.
-__device__ void rmc_path_props(const RmcK& k, const double* Pr, int kf, int kl, double* out,
-                               int lane) {
-  const int nb = k.ntau - 2 * k.ntauskip;
-  for (int ip = lane; ip < k.NP; ip += 64) {
-    double v = 0.0;
-    for (int i = 1 + k.ntauskip; i <= k.ntau - k.ntauskip; i++)
-      v = v + Pr[(int64_t)rslot(kf + i - 1, k.nbuf) * k.NP + ip];
-    v = v / (double)nb;
-    if (ip == k.jetot) v = 0.5 * (Pr[(int64_t)kl * k.NP + k.jetot] + Pr[(int64_t)kf * k.NP + k.jetot]);
-    out[ip] = v;
+// path_props scalars (reptation.f:1566-1577) for the window starting at slot
+// kf: mean over beads ntauskip+1..ntau-ntauskip, E from the two end beads.
+// One wavefront per (reptile, 64 properties); each lane adds its property's
+// beads in the reference's order, with RMC_PB loads in flight at a time.
+// With mv the window is the one the move slides to (kf, kl from ends and the
+// move), else slots 0..ntau-1.
+#define RMC_PB 16
+__global__ __launch_bounds__(64) void k_rmc_path(RmcK k, const double* __restrict__ P,
+                                                 const int32_t* __restrict__ ends,
+                                                 const int32_t* __restrict__ mv,
+                                                 double* __restrict__ out) {
+  const int r = blockIdx.x, ip = blockIdx.y * 64 + threadIdx.x;
+  const int nbuf = k.nbuf, NP = k.NP;
+  if (ip >= NP) return;
+  int kf = 0, kl = k.ntau - 1;
+  if (mv) {
+    const int idir = mv[3 * r], nd = mv[3 * r + 1];
+    kf = rslot(ends[2 * r] + idir * nd, nbuf);
+    kl = rslot(ends[2 * r + 1] + idir * nd, nbuf);
   }
+  const double* Pr = P + (int64_t)r * nbuf * NP;
+  const int nb = k.ntau - 2 * k.ntauskip;
+  int sl = rslot(kf + k.ntauskip, nbuf);
+  double v = 0.0;
+  for (int i0 = 0; i0 < nb; i0 += RMC_PB) {
+    double t[RMC_PB];
+#pragma unroll
+    for (int q = 0; q < RMC_PB; q++) {
+      t[q] = i0 + q < nb ? Pr[(int64_t)sl * NP + ip] : 0.0;
+      sl = sl + 1 == nbuf ? 0 : sl + 1;
+    }
+#pragma unroll
+    for (int q = 0; q < RMC_PB; q++)
+      if (i0 + q < nb) v = v + t[q];
+  }
+  v = v / (double)nb;
+  if (ip == k.jetot) v = 0.5 * (Pr[(int64_t)kl * NP + k.jetot] + Pr[(int64_t)kf * NP + k.jetot]);
+  out[(int64_t)r * NP + ip] = v;
 }
 
-// End of the move: path averages of the slid window, path weight
-// (reptation.f:2105-2160), reptation_metropolis_test and averages(2).
+// End of the move: path weight (reptation.f:2105-2160),
+// reptation_metropolis_test and averages(2), with k_rmc_path's averages of
+// the slid window.
 // One wavefront per reptile.
 __global__ __launch_bounds__(64) void k_rmc_finish(
     RmcK k, const double* __restrict__ P, const double* __restrict__ W, int32_t* __restrict__ ends,
     const int32_t* __restrict__ mv, int64_t* __restrict__ rng, double* __restrict__ rage,
-    double* __restrict__ pp_old, double* __restrict__ ppn, double* __restrict__ blk,
+    double* __restrict__ pp_old, const double* __restrict__ ppn, double* __restrict__ blk,
     double* __restrict__ trace) {
   const int r = blockIdx.x;
   const int lane = threadIdx.x;
@@ -248,9 +306,7 @@ __global__ __launch_bounds__(64) void k_rmc_finish(
   const int kf = rslot(jf + idir * nd, nbuf), kl = rslot(jl + idir * nd, nbuf);
   const double* Pr = P + (int64_t)r * nbuf * NP;
   const double* Wr = W + (int64_t)r * nbuf * RW_N;
-  double* pn = ppn + (int64_t)r * NP;
-  rmc_path_props(k, Pr, kf, kl, pn, lane);
-  __syncthreads();
+  const double* pn = ppn + (int64_t)r * NP;  // k_rmc_path's averages of the slid window
   __shared__ int s_acc;
   if (lane == 0) {
     const double m = k.mezzo, h = k.half, o = k.one, on = k.one_nds;
@@ -347,20 +403,18 @@ __global__ void k_rmc_scatter(RmcK k, int b, double* __restrict__ G, double* __r
   }
 }
 
-// read_path after the beads: path averages of slots 0..ntau-1, then the link
-// weights and Langevin probabilities of every link (get_w_stack,
+// read_path after the beads and k_rmc_path's averages of slots 0..ntau-1:
+// the link weights and Langevin probabilities of every link (get_w_stack,
 // reptation.f:1804-1851, and reptation.f:1789-1798).  One wavefront per
 // reptile.
 __global__ __launch_bounds__(64) void k_rmc_links(RmcK k, const double* __restrict__ X,
                                                   const double* __restrict__ G, double* __restrict__ P,
                                                   double* __restrict__ W, int32_t* __restrict__ ends,
-                                                  double* __restrict__ rage, double* __restrict__ pp_old) {
+                                                  double* __restrict__ rage) {
   const int r = blockIdx.x;
   const int lane = threadIdx.x;
   const int N = k.N, NP = k.NP, nbuf = k.nbuf;
   double* Pr = P + (int64_t)r * nbuf * NP;
-  rmc_path_props(k, Pr, 0, k.ntau - 1, pp_old + (int64_t)r * NP, lane);
-  __syncthreads();
   double* Wr = W + (int64_t)r * nbuf * RW_N;
   for (int i = 1; i < k.ntau; i++) {
     const int j = i - 1, jn = i;
@@ -527,8 +581,10 @@ extern "C" int qmc_rmc_start(qmc_ctx* c, const qmc_reptiles* r, const qmc_rmc_pa
     hipLaunchKernelGGL(k_rmc_scatter, dim3(nr), dim3(64), 0, st, k, b, r->g, r->props, r->age,
                        c->d_gn, c->d_pn, c->d_stn, c->d_sticky);
   }
+  hipLaunchKernelGGL(k_rmc_path, dim3(nr, (k.NP + 63) / 64), dim3(64), 0, st, k, r->props,
+                     r->ends, nullptr, r->pp_old);
   hipLaunchKernelGGL(k_rmc_links, dim3(nr), dim3(64), 0, st, k, r->x, r->g, r->props, r->w,
-                     r->ends, r->rage, r->pp_old);
+                     r->ends, r->rage);
   HIPCHK(hipGetLastError());
   return QMC_OK;
 }
@@ -556,14 +612,17 @@ extern "C" int qmc_rmc_sweep(qmc_ctx* c, const qmc_reptiles* r, const qmc_rmc_pa
     hipLaunchKernelGGL(k_rmc_begin, dim3((nr + 63) / 64), dim3(64), 0, st, k, r->ends, r->rng,
                        c->d_rmv);
     for (int id = 1; id <= prm->mdelta; id++) {
-      hipLaunchKernelGGL(k_rmc_propose, dim3((nr + 63) / 64), dim3(64), 0, st, k, id, r->x, r->g,
-                         c->d_rmv, r->rng, c->d_xn, sc, r->grad2_ph, g2n);
+      hipLaunchKernelGGL(k_rmc_propose, dim3((nr + wpb - 1) / wpb), dim3(QMC_BLOCK),
+                         sizeof(double) * 3 * 2 * k.N * wpb, st, k, id, r->x, r->g, c->d_rmv,
+                         r->rng, c->d_xn, sc, r->grad2_ph, g2n);
       rc = launch_eval(c, nr, c->d_xn, c->d_pn, c->d_gn, c->d_hn, g2n, c->d_stn, st);
       if (rc) return rc;
       hipLaunchKernelGGL(k_rmc_step, dim3((nr + wpb - 1) / wpb), dim3(QMC_BLOCK), 0, st, k, id,
                          r->x, r->g, r->props, r->age, r->w, c->d_rmv, r->rng, c->d_xn, c->d_gn,
                          c->d_pn, sc, r->grad2_ph, g2n, c->d_stn, c->d_sticky, tr);
     }
+    hipLaunchKernelGGL(k_rmc_path, dim3(nr, (k.NP + 63) / 64), dim3(64), 0, st, k, r->props,
+                       r->ends, c->d_rmv, ppn);
     hipLaunchKernelGGL(k_rmc_finish, dim3(nr), dim3(64), 0, st, k, r->props, r->w, r->ends,
                        c->d_rmv, r->rng, r->rage, r->pp_old, ppn, r->blk, tr);
   }
