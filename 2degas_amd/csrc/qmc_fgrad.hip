@@ -1,5 +1,5 @@
 // qmc_fgrad.hip -- drift and diagonal Laplacian terms of the backflow
-// determinants for 33..64 particles per spin (reptation.f:10195-10240).
+// determinants for up to 64 particles per spin (reptation.f:10195-10240).
 //
 // The per-orbital factors d_j = F_b(j,j) and diag(A^-1 G_a G_b A)_j =
 // sum_k F_a(j,k) F_b(k,j) (A^-1 G_a G_b A = F_a F_b) come from the

@@ -1,5 +1,6 @@
-// qmc_gjm.hip -- determinant path for 33..64 particles per spin on the FP64
-// matrix cores.
+// qmc_gjm.hip -- determinant path for up to 64 particles per spin on the
+// FP64 matrix cores (the default at every size; k_det is the LDS LU
+// alternative).
 //
 // k_gjm computes F_b = A^-1 (G_b A) (zv of reptation.f:10182-10194, with
 // A(i,j) = exp(i g_i . q_j), G_b = diag(i g^b)) and log|det A| in two phases,

@@ -53,7 +53,7 @@ struct qmc_ctx {
   int kc = 16;
   size_t lds_pairs = 0, lds_det = 0, lds_cross = 0;
   int tpp_det = 0;
-  bool use_gj = false;   // matrix-core Gauss-Jordan determinant path (33..64 per spin)
+  bool use_gj = false;   // matrix-core Gauss-Jordan determinant path (else k_det)
   size_t lds_gjm = 0;
   size_t lds_fgrad = 0;
   // per-kernel event timing

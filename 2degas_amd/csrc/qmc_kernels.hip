@@ -1,8 +1,9 @@
 // qmc_kernels.hip -- FP64 walker-evaluation kernels for gfx950.
 //
 // One local-energy evaluation (compute_properties(1), reptation.f:2661-2727)
-// of a batch of walkers is four launches (five above 32 particles per spin,
-// where k_gjm in qmc_gjm.hip and k_fgrad in qmc_fgrad.hip take k_det's part):
+// of a batch of walkers is five launches: k_pairs, k_gjm and k_fgrad
+// (qmc_gjm.hip, qmc_fgrad.hip: the determinant on the matrix cores), k_cross
+// and k_final; with QMC_NO_GJ set, k_det takes the part of k_gjm and k_fgrad:
 //   k_pairs   one workgroup per walker: pair geometry, u2 and u3 Jastrow
 //             (two_body/three_body/dotg, reptation.f:2777-3046), backflow
 //             quasi-coordinates (reptation.f:10023-10058), real-space Ewald

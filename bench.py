@@ -27,16 +27,21 @@ FP64_PEAK_TFLOPS = 78.6
 KERNELS = ["pairs", "det", "fgrad", "cross", "final"]
 
 
+def gj_path():
+    """The library's determinant path: k_gjm unless QMC_NO_GJ is set."""
+    return os.environ.get("QMC_NO_GJ") is None
+
+
 def kernel_flops(nup, ndown, nk_ewald):
     """Algorithmic flops per walker of each kernel (pairs, determinant,
     drift/diagonal terms, cross, final); their sum is qmc_flops_per_eval.
-    The determinant is k_gjm (matrix cores) above 32 particles per spin
-    (k_fgrad then does the diagonal terms) and k_det, which does both, below;
-    the determinant's count is the Gauss-Jordan solve's, whichever kernel
-    runs it."""
+    The determinant is k_gjm (matrix cores; k_fgrad then does the diagonal
+    terms) unless QMC_NO_GJ selects k_det, which does both; the
+    determinant's count is the Gauss-Jordan solve's, whichever kernel runs
+    it."""
     n = nup + ndown
     pairs = 160.0 * n * (n - 1) / 2 + 5.0 * n * nk_ewald + 20.0 * nk_ewald
-    gj = max(nup, ndown) > 32
+    gj = gj_path()
     det = sum(24.0 * m ** 3 + (0.0 if gj else 37.0 * m * m) for m in (nup, ndown))
     fgrad = sum(37.0 * m * m for m in (nup, ndown)) if gj else 0.0
     cross = sum(24.0 * n * m * m for m in (nup, ndown))
@@ -181,7 +186,7 @@ def main():
     guard = int((st.cpu().numpy() & 1).sum())
     kf = kernel_flops(args.nup, args.ndown, info["nk_ewald"])
     names = KERNELS
-    gj = max(args.nup, args.ndown) > 32
+    gj = gj_path()
     det_kernel = "k_gjm" if gj else "k_det"
     kname = {"pairs": "k_pairs", "det": det_kernel, "fgrad": "k_fgrad",
              "cross": "k_cross", "final": "k_final"}

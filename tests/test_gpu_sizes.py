@@ -1,9 +1,9 @@
-"""Parity with the oracle across particle counts: every size of the
-matrix-core determinant (33..64 per spin, ragged and padded tiles, one or
-two spins above 32), the round-robin pair schedule with one type, two
-equal types and two unequal types, and the LDS determinant below 33.
-Decks come from the generator (rs = 5, 40 Ewald shells); tolerance 1e-10
-relative, as for the reference goldens."""
+"""Parity with the oracle across particle counts: the matrix-core
+determinant from 3 to 64 per spin (ragged and padded tiles, one or two
+spins above 32), the round-robin pair schedule with one type, two equal
+types and two unequal types, and the LDS LU determinant k_det that
+QMC_NO_GJ selects.  Decks come from the generator (rs = 5, 40 Ewald
+shells); tolerance 1e-10 relative, as for the reference goldens."""
 import os
 
 import numpy as np
@@ -14,12 +14,12 @@ from conftest import compare_eval, pkg
 pytestmark = pytest.mark.gpu
 
 RTOL = 1e-10
-SIZES = [(33, 33), (37, 29), (40, 20), (48, 47), (57, 0), (64, 64), (64, 1), (20, 12)]
+SIZES = [(33, 33), (37, 29), (40, 20), (48, 47), (57, 0), (64, 64), (64, 1), (20, 12),
+         (29, 29), (13, 13), (5, 3)]
+LU_SIZES = [(20, 12), (29, 29), (37, 29)]
 
 
-@pytest.mark.parametrize("nup,ndown", SIZES)
-def test_sizes_match_oracle(nup, ndown, tmp_path, oracle_mod):
-    print(f"\n[gpu] sizes {nup}+{ndown}", flush=True)
+def _sizes_case(nup, ndown, tmp_path, oracle_mod, tag):
     dg, deck, hip = pkg("deckgen"), pkg("deck"), pkg("hip")
     path = os.path.join(str(tmp_path), "deck")
     dg.make_deck(path, "sz", nup, ndown, 5.0, 40)
@@ -35,9 +35,23 @@ def test_sizes_match_oracle(nup, ndown, tmp_path, oracle_mod):
         out = ctx.eval(x)
     finally:
         ctx.close()
-    errs = compare_eval(out, gold, ns, RTOL, f"{nup}+{ndown}")
+    errs = compare_eval(out, gold, ns, RTOL, f"{tag} {nup}+{ndown}")
     print("   max err", max(errs.values()), flush=True)
     assert np.array_equal(out["status"], ref["status"])
+
+
+@pytest.mark.parametrize("nup,ndown", SIZES)
+def test_sizes_match_oracle(nup, ndown, tmp_path, oracle_mod):
+    print(f"\n[gpu] sizes {nup}+{ndown}", flush=True)
+    _sizes_case(nup, ndown, tmp_path, oracle_mod, "gj")
+
+
+@pytest.mark.parametrize("nup,ndown", LU_SIZES)
+def test_lu_determinant_matches_oracle(nup, ndown, tmp_path, oracle_mod, monkeypatch):
+    """k_det (LINPACK LU and inverse in LDS), read at context creation."""
+    print(f"\n[gpu] LU determinant {nup}+{ndown}", flush=True)
+    monkeypatch.setenv("QMC_NO_GJ", "1")
+    _sizes_case(nup, ndown, tmp_path, oracle_mod, "lu")
 
 
 def test_phases_off_lattice_match_oracle(tmp_path, oracle_mod):
