@@ -138,8 +138,9 @@ __device__ __forceinline__ void gm_pivot_steps(GmShared& sh, zd (&P)[4], zd (&Fp
   }
 }
 
-__global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, double* __restrict__ ws,
-                                                       double* __restrict__ Fbuf) {
+template <bool PART>
+__device__ __forceinline__ void gjm_run(DevSys s, WsLayout L, double* __restrict__ ws,
+                                        double* __restrict__ Fbuf) {
   extern __shared__ __attribute__((aligned(16))) unsigned char gm_lds[];
   GmShared& sh = *reinterpret_cast<GmShared*>(gm_lds);
   const int w = blockIdx.x, sp = blockIdx.y;
@@ -152,6 +153,14 @@ __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, dou
   // MFMA lane roles: accumulator rows lr + 4v, column lc; operand k index lr
   const int lr = lane >> 4, lc = lane & 15;
   const int col = 16 * ct + lc;
+  // PART: tiles wholly beyond row or column m start at zero and stay zero
+  // (the row operations of rows >= m and the pivot rows' entries in columns
+  // >= m are zero), and nothing is read from them for rows or columns < m:
+  // their products are skipped.  live2: both of the wavefront's row tiles
+  // take part, live1: the first only.
+  const int ntile = (m + 15) >> 4;
+  const bool live1 = !PART || (ct < ntile && 2 * rh < ntile);
+  const bool live2 = !PART || (live1 && 2 * rh + 1 < ntile);
   const double* qw = ws + (int64_t)w * L.per_walker + L.off_q;
   for (int e = tid; e < 64; e += GM_THREADS) {
     const bool in = e < m;
@@ -236,16 +245,21 @@ __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, dou
     zd fo[2];
 #pragma unroll
     for (int t = 0; t < 2; t++) fo[t] = sh.Fl[(16 * (2 * rh + t) + lc) * GM_PS + lr];
+    auto upd = [&](auto ntc) {
+      constexpr int NT = decltype(ntc)::value;
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-      acc[t][0] = gm_mfma(fo[t].x, uo.x, acc[t][0]);
-      acc[t][1] = gm_mfma(fo[t].x, uo.y, acc[t][1]);
-    }
+      for (int t = 0; t < NT; t++) {
+        acc[t][0] = gm_mfma(fo[t].x, uo.x, acc[t][0]);
+        acc[t][1] = gm_mfma(fo[t].x, uo.y, acc[t][1]);
+      }
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-      acc[t][0] = gm_mfma(-fo[t].y, uo.y, acc[t][0]);
-      acc[t][1] = gm_mfma(fo[t].y, uo.x, acc[t][1]);
-    }
+      for (int t = 0; t < NT; t++) {
+        acc[t][0] = gm_mfma(-fo[t].y, uo.y, acc[t][0]);
+        acc[t][1] = gm_mfma(fo[t].y, uo.x, acc[t][1]);
+      }
+    };
+    if (live2) upd(std::integral_constant<int, 2>{});
+    else if (live1) upd(std::integral_constant<int, 1>{});
     gm_put_tiles(sh, acc, rh, col, lr);
     __syncthreads();
   }
@@ -276,7 +290,9 @@ __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, dou
   for (int t = 0; t < 2; t++)
 #pragma unroll
     for (int b = 0; b < 2; b++) h[t][b][0] = h[t][b][1] = gm_d4{0.0, 0.0, 0.0, 0.0};
-  for (int ks = 0; 4 * ks < m; ks++) {
+  // a wavefront with no live tile has neither outputs nor diagonal terms
+  // (its particles are >= m)
+  for (int ks = 0; live1 && 4 * ks < m; ks++) {
     const int kp = 4 * ks + lr;
     double bx_re = 0.0, bx_im = 0.0, by_re = 0.0, by_im = 0.0;
     if (kp < m && j < m) {
@@ -365,6 +381,19 @@ __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, dou
       dsc[DS_SING] = sh.sing ? 1.0 : 0.0;
     }
   }
+}
+
+// every spin 49..64 particles: all 4 x 4 tiles take part
+__global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, double* __restrict__ ws,
+                                                       double* __restrict__ Fbuf) {
+  gjm_run<false>(s, L, ws, Fbuf);
+}
+
+// a spin of 1..48 particles: the products of the tiles beyond m are skipped
+__global__ __launch_bounds__(GM_THREADS, 4) void k_gjm_part(DevSys s, WsLayout L,
+                                                            double* __restrict__ ws,
+                                                            double* __restrict__ Fbuf) {
+  gjm_run<true>(s, L, ws, Fbuf);
 }
 
 size_t gjm_lds_bytes() { return sizeof(GmShared); }

@@ -54,6 +54,7 @@ struct qmc_ctx {
   size_t lds_pairs = 0, lds_det = 0, lds_cross = 0;
   int tpp_det = 0;
   bool use_gj = false;   // matrix-core Gauss-Jordan determinant path (else k_det)
+  bool gjm_part = false; // a spin below 49 particles: k_gjm_part, which skips the tiles beyond m
   size_t lds_gjm = 0;
   size_t lds_fgrad = 0;
   // per-kernel event timing

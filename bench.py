@@ -187,7 +187,8 @@ def main():
     kf = kernel_flops(args.nup, args.ndown, info["nk_ewald"])
     names = KERNELS
     gj = gj_path()
-    det_kernel = "k_gjm" if gj else "k_det"
+    part = any(0 < m <= 48 for m in (args.nup, args.ndown))
+    det_kernel = ("k_gjm_part" if part else "k_gjm") if gj else "k_det"
     kname = {"pairs": "k_pairs", "det": det_kernel, "fgrad": "k_fgrad",
              "cross": "k_cross", "final": "k_final"}
     dom = int(np.argmax(ms_k))
