@@ -71,7 +71,8 @@ struct qmc_ctx {
   double* d_rsc = nullptr;
 };
 
-// Queue one batched evaluation (k_pairs, k_det, k_cross, k_final) on st.
+// Queue one batched evaluation (k_pairs, k_gjm or k_gjm_part and k_fgrad, or
+// k_det, then k_cross, k_final) on st.
 int launch_eval(qmc_ctx* c, int nw, const double* d_x, double* d_props, double* d_g, double* d_h,
                 double* d_g2, int32_t* d_status, hipStream_t st);
 void free_vmc(qmc_ctx* c);

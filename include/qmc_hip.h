@@ -281,8 +281,9 @@ int qmc_rmc_sweep(qmc_ctx* ctx, const qmc_reptiles* r, const qmc_rmc_params* prm
  * which has no timers).  qmc_timing(ctx, 1) resets and enables,
  * qmc_timing(ctx, 0) disables.  qmc_timing_read waits for the recorded work
  * and returns the mean milliseconds per launch of the kernels in the order
- * pairs, determinant (k_gjm or k_det), drift and diagonal terms (k_fgrad; 0 when
- * k_det does them), cross term, final (ms[5]) and the launches counted.
+ * pairs, determinant (k_gjm, k_gjm_part or k_det), drift and diagonal terms
+ * (k_fgrad; 0 when k_det does them), cross term, final (ms[5]) and the
+ * launches counted.
  */
 int qmc_timing(qmc_ctx* ctx, int enable);
 int qmc_timing_read(qmc_ctx* ctx, double* ms, int* nlaunch);
