@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 RTOL = 1e-10
 SIZES = [(33, 33), (37, 29), (40, 20), (48, 47), (57, 0), (64, 64), (64, 1), (20, 12),
-         (29, 29), (13, 13), (5, 3)]
+         (29, 29), (13, 13), (5, 3), (37, 0)]
 LU_SIZES = [(20, 12), (29, 29), (37, 29)]
 
 
