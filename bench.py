@@ -6,7 +6,12 @@ N=122 (61+61) electrons, rs=10, Slater-Jastrow-backflow with u2 + u3,
 Ewald over the first 200 k-shells (900 k-vectors), 4096 walkers per launch,
 synthetic positions (a jittered 12x12 grid per walker) and a generated deck.
 
-Prints one JSON line (see DESIGN.md for the fields).
+Prints one JSON line: the throughput (value, ms_per_step), the workload
+(config), the dominant kernel's roofline (algorithmic TFLOP/s against the
+FP64 matrix-core peak, HBM bytes per launch from the committed counter
+summary), the per-kernel HIP-event means (kernel_ms) and the reference's
+CPU rate on this host (cpu_baseline).  --rqmc and --serve measure the
+reptation and served-job workloads instead.
 """
 import argparse
 import importlib
