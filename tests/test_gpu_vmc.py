@@ -66,3 +66,35 @@ def test_vmc_block_sums_accumulate_old_properties():
     tot = sum(v.cpu().numpy() for v in e)
     assert np.allclose(wk.blk[:, 0].cpu().numpy(), tot, rtol=1e-14, atol=0)
     assert np.all(wk.blk[:, 2].cpu().numpy() <= 3.0)
+
+
+def test_vmc_other_draw_order_matches_oracle(oracle_mod):
+    """bm_order = 1 (the Box-Muller line's two rannyu() calls evaluated in
+    the other order): from the reference's recorded walkers and RNG states,
+    the device's accept/reject decisions and acceptance probabilities match
+    the oracle's replay in the same order."""
+    print("\n[gpu] vmc draw order 1 vs oracle", flush=True)
+    import torch
+    hip = pkg("hip")
+    abi = pkg("abi")
+    gold = load_golden("replay_n26.npz")
+    d = load_deck("n26")
+    nw, nsteps, delta = 4, 40, float(gold["delta"])
+    x0 = np.asarray(gold["ck_x"][:nw])
+    seeds = [np.asarray(s) for s in gold["ck_seeds"][:nw]]
+    ctx = hip.Context(d, twist=(1, 1))
+    wk = hip.Walkers(ctx, x0, [abi.limbs_to_state(s) for s in seeds])
+    wk.start()
+    tp, ta = wk.sweep(nsteps, delta, bm_order=1, trace=True)
+    torch.cuda.synchronize()
+    assert np.all(wk.status() == 0)
+    p, age = tp.cpu().numpy(), ta.cpu().numpy()
+    o = oracle_mod.Oracle(d, twist=(1, 1))
+    mism, worst = 0, 0.0
+    for w in range(nw):
+        r = o.vmc_replay(x0[w], nsteps, delta, seeds[w], 1)
+        mism += int(np.sum((age[:, w] == 0) != (r["age"] == 0)))
+        worst = max(worst, float(np.max(np.abs(p[:, w] - r["p"]) / np.maximum(r["p"], 1e-300))))
+    print(f"   moves {nw * nsteps} mismatches {mism} max rel p {worst:.2e}", flush=True)
+    assert mism == 0
+    assert worst < 1e-8
