@@ -50,6 +50,7 @@ struct qmc_ctx {
   double *d_x = nullptr, *d_props = nullptr, *d_g = nullptr, *d_h = nullptr, *d_g2 = nullptr;
   int32_t* d_status = nullptr;
   hipStream_t stream = nullptr;
+  double delta = 0.0;  // current time step (COMMON delta) for the drift cutoff
   int kc = 16;
   size_t lds_pairs = 0, lds_det = 0, lds_cross = 0;
   int tpp_det = 0;
@@ -73,8 +74,9 @@ struct qmc_ctx {
 
 // Queue one batched evaluation (k_pairs, k_gjm or k_gjm_part and k_fgrad, or
 // k_det, then k_cross, k_final) on st.
+// delta is the time step the drift cutoff of kinetic uses (reptation.f:2755).
 int launch_eval(qmc_ctx* c, int nw, const double* d_x, double* d_props, double* d_g, double* d_h,
-                double* d_g2, int32_t* d_status, hipStream_t st);
+                double* d_g2, int32_t* d_status, hipStream_t st, double delta);
 void free_vmc(qmc_ctx* c);
 int ensure_vmc_scratch(qmc_ctx* c, int nw);
 void free_rmc(qmc_ctx* c);

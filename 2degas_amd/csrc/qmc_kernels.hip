@@ -772,13 +772,15 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_det(DevSys s, WsLayout L,
 }
 
 // ---------------------------------------------------------------------------
-// k_final: one wavefront per walker.
+// k_final: one wavefront per walker.  delta is the current time step (the
+// COMMON delta that kinetic's drift cutoff reads, reptation.f:2755).
 __global__ __launch_bounds__(QMC_BLOCK) void k_final(DevSys s, WsLayout L, double* __restrict__ ws,
                                                      double* __restrict__ props,
                                                      double* __restrict__ gout,
                                                      double* __restrict__ hout,
                                                      double* __restrict__ grad2,
-                                                     int32_t* __restrict__ status, int nw) {
+                                                     int32_t* __restrict__ status, int nw,
+                                                     double delta) {
   const int w = blockIdx.x * (QMC_BLOCK / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (w >= nw) return;
@@ -824,9 +826,10 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_final(DevSys s, WsLayout L, doubl
     const int t = ptype(s, i);
     const double x2 = gx * gx + gy * gy;
     if (s.adrift != 0.0) {
-      // drift cutoff (kinetic, reptation.f:2747-2760)
-      const double vari = 1.0 / (2.0 * pick2(s.hbs2m, t) * s.delta);
-      double gg = x2 * s.adrift * s.delta * s.delta * vari;
+      // drift cutoff (kinetic, reptation.f:2747-2760), vari as sigma sets it
+      // (reptation.f:855-870); every particle here has hbs2m != 0
+      const double vari = 1.0 / (2.0 * pick2(s.hbs2m, t) * delta);
+      double gg = x2 * s.adrift * delta * delta * vari;
       gg = (-1.0 + sqrt(1.0 + 2.0 * gg)) / gg;
       ggs += gg;
       gx *= gg;
@@ -846,11 +849,12 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_final(DevSys s, WsLayout L, doubl
 #pragma unroll
     for (int t = 0; t < 2; t++) {
       if (t >= s.ntypes) break;
+      hout[(int64_t)w * 2 + t] = h[t];
+      if (s.hbs2m[t] == 0.0) continue;  // no kinetic slots (reptation.f:2739)
       const double kin = s.hbs2m[t] * (h[t] - g2t[t]);
       kin0 += kin;
       p[s.jkin[t]] = kin / s.np[t];
       p[s.jun[t]] = 1.0 / sqrt(g2t[t]);
-      hout[(int64_t)w * 2 + t] = h[t];
     }
     if (s.ntypes == 1) hout[(int64_t)w * 2 + 1] = 0.0;
     p[s.jkin0] = kin0 / s.npnorm;
@@ -861,7 +865,7 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_final(DevSys s, WsLayout L, doubl
     p[s.jltf] = ltf;
 #pragma unroll
     for (int t = 0; t < 2; t++)
-      if (t < s.ntypes) p[s.jpot[t]] = (s.tailc[t] + sc[SC_P0 + t]) / s.np[t];
+      if (t < s.ntypes && s.jpot[t] >= 0) p[s.jpot[t]] = (s.tailc[t] + sc[SC_P0 + t]) / s.np[t];
     p[s.jpot0] = (s.tailc_tot + sc[SC_PT] + s.v0) / s.npnorm;
     p[s.jetot] = p[s.jpot0] + p[s.jkin0] + p[s.jefp];
     p[s.je2] = p[s.jetot] * p[s.jetot];

@@ -511,6 +511,8 @@ static int rmc_setup(qmc_ctx* c, const qmc_reptiles* r, const qmc_rmc_params* pr
   for (int t = 0; t < s.ntypes; t++)
     if (s.np[t] > 0 && s.hbs2m[t] == 0.0)
       return arg_fail("reptation needs every populated type to move (hbs2m != 0)");
+  // the rmc record sets the COMMON delta before sigma (reptation.f:795-805)
+  c->delta = prm->delta;
   k = RmcK{};
   k.nr = r->nr;
   k.ntau = r->ntau;
@@ -576,7 +578,8 @@ extern "C" int qmc_rmc_start(qmc_ctx* c, const qmc_reptiles* r, const qmc_rmc_pa
   HIPCHK(hipMemsetAsync(c->d_sticky, 0, sizeof(int32_t) * nr, st));
   for (int b = 0; b < r->ntau; b++) {
     hipLaunchKernelGGL(k_rmc_gather, dim3(nr), dim3(64), 0, st, k, b, r->x, c->d_xn);
-    rc = launch_eval(c, nr, c->d_xn, c->d_pn, c->d_gn, c->d_hn, r->grad2_ph, c->d_stn, st);
+    rc = launch_eval(c, nr, c->d_xn, c->d_pn, c->d_gn, c->d_hn, r->grad2_ph, c->d_stn, st,
+                     prm->delta);
     if (rc) return rc;
     hipLaunchKernelGGL(k_rmc_scatter, dim3(nr), dim3(64), 0, st, k, b, r->g, r->props, r->age,
                        c->d_gn, c->d_pn, c->d_stn, c->d_sticky);
@@ -615,7 +618,7 @@ extern "C" int qmc_rmc_sweep(qmc_ctx* c, const qmc_reptiles* r, const qmc_rmc_pa
       hipLaunchKernelGGL(k_rmc_propose, dim3((nr + wpb - 1) / wpb), dim3(QMC_BLOCK),
                          sizeof(double) * 3 * 2 * k.N * wpb, st, k, id, r->x, r->g, c->d_rmv,
                          r->rng, c->d_xn, sc, r->grad2_ph, g2n);
-      rc = launch_eval(c, nr, c->d_xn, c->d_pn, c->d_gn, c->d_hn, g2n, c->d_stn, st);
+      rc = launch_eval(c, nr, c->d_xn, c->d_pn, c->d_gn, c->d_hn, g2n, c->d_stn, st, prm->delta);
       if (rc) return rc;
       hipLaunchKernelGGL(k_rmc_step, dim3((nr + wpb - 1) / wpb), dim3(QMC_BLOCK), 0, st, k, id,
                          r->x, r->g, r->props, r->age, r->w, c->d_rmv, r->rng, c->d_xn, c->d_gn,

@@ -185,7 +185,7 @@ extern "C" int qmc_vmc_start(qmc_ctx* c, const qmc_walkers* w, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   rc = ensure_vmc_scratch(c, w->nw);
   if (rc) return rc;
-  rc = launch_eval(c, w->nw, w->x, w->props, w->g, w->h, w->grad2_ph, c->d_stn, st);
+  rc = launch_eval(c, w->nw, w->x, w->props, w->g, w->h, w->grad2_ph, c->d_stn, st, c->delta);
   if (rc) return rc;
   hipLaunchKernelGGL(k_vmc_start, dim3((w->nw + 255) / 256), dim3(256), 0, st, w->nw,
                      c->info.n_props, c->info.jacc - 1, w->props, w->age, c->d_sticky);
@@ -205,6 +205,8 @@ extern "C" int qmc_vmc_sweep(qmc_ctx* c, const qmc_walkers* w, int nsteps, doubl
   rc = ensure_vmc_scratch(c, w->nw);
   if (rc) return rc;
   const DevSys& s = c->s;
+  // the vmc record sets the COMMON delta before sigma (reptation.f:740-746)
+  c->delta = delta;
   // sigma (reptation.f:855-870)
   const double var0 = 2.0 * s.hbs2m[0] * delta;
   const double var1 = s.ntypes > 1 ? 2.0 * s.hbs2m[1] * delta : 1.0;
@@ -213,7 +215,7 @@ extern "C" int qmc_vmc_sweep(qmc_ctx* c, const qmc_walkers* w, int nsteps, doubl
     hipLaunchKernelGGL(k_vmc_propose, dim3((nw + 63) / 64), dim3(64), 0, st, s, nw, w->x, w->g,
                        c->d_xn, w->rng, c->d_wold, var0, var1, bm_order);
     // the fixed-phase scratch is shared by proposals, as the COMMON is
-    rc = launch_eval(c, nw, c->d_xn, c->d_pn, c->d_gn, c->d_hn, w->grad2_ph, c->d_stn, st);
+    rc = launch_eval(c, nw, c->d_xn, c->d_pn, c->d_gn, c->d_hn, w->grad2_ph, c->d_stn, st, delta);
     if (rc) return rc;
     const int wpb = QMC_BLOCK / 64;
     hipLaunchKernelGGL(k_vmc_accept, dim3((nw + wpb - 1) / wpb), dim3(QMC_BLOCK), 0, st, s, nw,

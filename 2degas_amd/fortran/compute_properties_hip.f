@@ -122,6 +122,8 @@ c g-vectors: re-send when tabc/twistg changed them
        rc = qmc_set_twist(ctx, gsent, ng)
        if (rc .ne. QMC_OK) stop 'compute_properties (GPU): g-vectors'
       endif
+c the drift cutoff reads the current COMMON delta (reptation.f:2755)
+      rc = qmc_set_delta(ctx, delta)
 c one walker through the C-ABI
       do i = 1, nptot
        xw(1,i) = x_new(1,i)

@@ -76,6 +76,12 @@ module qmc_hip_mod
        integer(c_int), value :: ng
      end function qmc_set_twist
 
+     integer(c_int) function qmc_set_delta(ctx, delta) bind(C, name='qmc_set_delta')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: delta
+     end function qmc_set_delta
+
      integer(c_int) function qmc_twist_gvec(kvec, nk, el, ntheta, ith, jth, gvec) &
           bind(C, name='qmc_twist_gvec')
        import :: c_int, c_double

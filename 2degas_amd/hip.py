@@ -18,7 +18,7 @@ EXPORTS = ("qmc_ctx_create", "qmc_ctx_destroy", "qmc_ctx_info", "qmc_set_twist",
            "qmc_twist_gvec", "qmc_eval_batch", "qmc_eval_batch_device", "qmc_strerror",
            "qmc_last_error", "qmc_flops_per_eval", "qmc_timing", "qmc_timing_read",
            "qmc_vmc_start", "qmc_vmc_sweep", "qmc_vmc_status", "qmc_rmc_start",
-           "qmc_rmc_sweep")
+           "qmc_rmc_sweep", "qmc_set_delta")
 
 _lib = None
 
@@ -52,6 +52,7 @@ def lib():
         L.qmc_ctx_destroy.argtypes = [vp]
         L.qmc_ctx_info.argtypes = [vp, C.POINTER(abi.QmcInfo)]
         L.qmc_set_twist.argtypes = [vp, dp, C.c_int]
+        L.qmc_set_delta.argtypes = [vp, C.c_double]
         L.qmc_twist_gvec.argtypes = [dp, C.c_int, dp, C.c_int, C.c_int, C.c_int, dp]
         L.qmc_eval_batch.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp, ip]
         L.qmc_eval_batch_device.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, C.c_int, vp]
@@ -112,6 +113,12 @@ class Context:
         rc = lib().qmc_set_twist(self.h, abi.dptr(self.gvec), self.gvec.shape[0])
         if rc:
             raise QmcError(rc, "qmc_set_twist")
+
+    def set_delta(self, delta):
+        """Time step read by the drift cutoff (decks with adrift != 0)."""
+        rc = lib().qmc_set_delta(self.h, float(delta))
+        if rc:
+            raise QmcError(rc, "qmc_set_delta")
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -204,7 +211,11 @@ class Walkers:
         st = stream if stream is not None else torch.cuda.current_stream()
         return C.c_void_p(st.cuda_stream)
 
-    def start(self, stream=None):
+    def start(self, stream=None, delta=None):
+        """read_conf's start; delta (the vmc record's time step) is what the
+        drift cutoff of an adrift deck reads during this evaluation."""
+        if delta is not None:
+            self.ctx.set_delta(delta)
         rc = lib().qmc_vmc_start(self.ctx.h, C.byref(self.s), self._stream(stream))
         if rc:
             raise QmcError(rc, "qmc_vmc_start")

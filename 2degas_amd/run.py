@@ -269,7 +269,7 @@ class Run:
                 x = self.read_configs()
                 states = [abi.limbs_to_state(l) for l in self.limbs]
                 wk = hip.Walkers(ctx, x, states)
-                wk.start()
+                wk.start(delta=delta)
                 for iblk in range(self.iblk0, nblk + 1):
                     if iblk == 1 or self.stats is None:
                         self.stats = _Stats(info["n_props"])

@@ -108,10 +108,10 @@ class _Running:
         states = [abi.limbs_to_state([s[0], s[1], s[2], (2 * (s[3] + k) + 1) % 4096])
                   for k in range(spec.nw)]
         self.stream = torch.cuda.Stream()
+        self.delta = spec.delta if spec.delta > 0 else spec.rs * spec.rs / self.deck.nptot
         with torch.cuda.stream(self.stream):
             self.walkers = hip.Walkers(self.ctx, x, states)
-            self.walkers.start(stream=self.stream)
-        self.delta = spec.delta if spec.delta > 0 else spec.rs * spec.rs / self.deck.nptot
+            self.walkers.start(stream=self.stream, delta=self.delta)
         self.nblk = 0
         self.blocks = []
         self.evals = 0

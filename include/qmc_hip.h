@@ -32,8 +32,8 @@ enum {
 };
 
 /* Compiled limits of the device path. */
-#define QMC_MAX_PARTICLES 256  /* total particles per walker                */
-#define QMC_MAX_ORBITALS 64    /* particles per spin in one determinant      */
+#define QMC_MAX_PARTICLES 128  /* total particles per walker (2 types x 64)  */
+#define QMC_MAX_ORBITALS 64    /* particles per type in one determinant      */
 #define QMC_MAX_TYPES 2
 
 /*
@@ -74,7 +74,8 @@ typedef struct qmc_tables {
   int32_t nrhok;
   int32_t ntheta;          /* twists per direction (0 = Gamma, no efp)     */
   double adrift;           /* drift cutoff (0 = off, reference default)    */
-  double delta;            /* time step used by the drift cutoff           */
+  double delta;            /* initial time step of the drift cutoff; later
+                              values come from qmc_set_delta and the sweeps */
 } qmc_tables;
 
 /* Sizes of a context (slot numbers are 1-based like the reference's). */
@@ -122,6 +123,16 @@ int qmc_set_twist(qmc_ctx* ctx, const double* gvec, int ng);
  */
 int qmc_twist_gvec(const double* kvec, int nk, const double* el, int ntheta,
                    int ith, int jth, double* gvec_out);
+
+/*
+ * Set the time step that the drift cutoff of `kinetic` reads when the deck
+ * sets adrift != 0: the COMMON delta of /c_sim/ that each driver record
+ * assigns before calling sigma (reptation.f:740-746, 795-805, 855-870,
+ * 2755).  qmc_eval_batch[_device] and qmc_vmc_start use the value set last;
+ * qmc_vmc_sweep and qmc_rmc_start/qmc_rmc_sweep set it from their own
+ * delta.  A Fortran forwarder passes its COMMON delta before every call.
+ */
+int qmc_set_delta(qmc_ctx* ctx, double delta);
 
 /*
  * Batched local-energy evaluation: for each of nw walkers, everything

@@ -13,28 +13,36 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libqmc_oracle.so")
+# the same restatement evaluated in long double (64-bit significand): the
+# exact value of the reference's formulas for the double inputs, against
+# which the rounding error of an FP64 result is measured
+LIB_LD = os.path.join(HERE, "libqmc_oracle_ld.so")
 
 _pkg = importlib.import_module("2degas_amd")
 abi = importlib.import_module("2degas_amd.abi")
 
 
 def build(force=False):
-    """Compile the C restatement with gcc (seconds)."""
+    """Compile the C restatement with gcc (seconds), in double and in long
+    double."""
     src = os.path.join(HERE, "qmc_oracle.c")
-    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(src):
-        subprocess.check_call(["gcc", "-O2", "-fPIC", "-shared", "-o", LIB, src, "-lm"])
+    for out, extra in ((LIB, []), (LIB_LD, ["-DORC_REAL=long double"])):
+        if force or not os.path.exists(out) or os.path.getmtime(out) < os.path.getmtime(src):
+            subprocess.check_call(["gcc", "-O2", "-fPIC", "-shared"] + extra +
+                                  ["-o", out, src, "-lm"])
     return LIB
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(extended=False):
+    """The oracle library (extended=True: the long-double build)."""
+    if extended not in _libs:
+        path = LIB_LD if extended else LIB
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         dp, ip = abi.c_double_p, abi.c_int32_p
         L.orc_create.restype = C.c_void_p
         L.orc_create.argtypes = [C.POINTER(abi.QmcTables), C.POINTER(C.c_int)]
@@ -45,9 +53,10 @@ def lib():
         L.orc_eval_batch.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp, dp, ip]
         L.orc_vmc_replay.argtypes = [C.c_void_p, dp, C.c_int, C.c_double, ip, C.c_int,
                                      dp, dp, dp, dp, dp, ip]
+        L.orc_set_delta.argtypes = [C.c_void_p, C.c_double]
         L.orc_rannyu.restype = C.c_double
-        _lib = L
-    return _lib
+        _libs[extended] = L
+    return _libs[extended]
 
 
 def twist_gvec(kvec, el, ntheta, ith, jth, variant=0):
@@ -61,27 +70,34 @@ def twist_gvec(kvec, el, ntheta, ith, jth, variant=0):
 
 
 class Oracle:
-    """compute_properties(1) on the CPU for a deck (one context)."""
+    """compute_properties(1) on the CPU for a deck (one context).  With
+    extended=True the sums run in long double: the exact value of the
+    reference's formulas for the same double inputs (the twist list is
+    still the double one, as the reference builds it)."""
 
-    def __init__(self, deck, twist=(1, 1), gvec=None):
+    def __init__(self, deck, twist=(1, 1), gvec=None, extended=False):
         self.deck = deck
+        self.L = lib(extended)
         s, self._keep = deck.pack()
         err = C.c_int(0)
-        self.h = lib().orc_create(C.byref(s), C.byref(err))
+        self.h = self.L.orc_create(C.byref(s), C.byref(err))
         if not self.h:
             raise ValueError(f"oracle rejected deck (code {err.value})")
         if gvec is None:
             gvec = twist_gvec(deck.kvec, deck.el, deck.ntheta, twist[0], twist[1], 0)
         self.gvec = np.ascontiguousarray(gvec, dtype=np.float64)
-        lib().orc_set_twist(self.h, abi.dptr(self.gvec), self.gvec.shape[0])
+        self.L.orc_set_twist(self.h, abi.dptr(self.gvec), self.gvec.shape[0])
         inf = abi.QmcInfo()
-        lib().orc_info(self.h, C.byref(inf))
+        self.L.orc_info(self.h, C.byref(inf))
         self.info = inf.as_dict()
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_destroy(self.h)
+            self.L.orc_destroy(self.h)
             self.h = None
+
+    def set_delta(self, delta):
+        self.L.orc_set_delta(self.h, float(delta))
 
     def eval(self, x, grad2_in=None):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, self.deck.nptot, 2)
@@ -91,7 +107,7 @@ class Oracle:
         h = np.zeros((nw, 2))
         g2 = np.zeros(nw) if grad2_in is None else np.array(grad2_in, dtype=np.float64)
         st = np.zeros(nw, np.int32)
-        rc = lib().orc_eval_batch(self.h, nw, abi.dptr(x), abi.dptr(props), abi.dptr(g),
+        rc = self.L.orc_eval_batch(self.h, nw, abi.dptr(x), abi.dptr(props), abi.dptr(g),
                                   abi.dptr(h), abi.dptr(g2), abi.iptr(st))
         if rc:
             raise RuntimeError(f"oracle eval failed: {rc}")
@@ -104,7 +120,7 @@ class Oracle:
         xf = np.zeros_like(x0)
         sd = np.ascontiguousarray(seed_limbs, dtype=np.int32)
         sf = np.zeros(4, np.int32)
-        rc = lib().orc_vmc_replay(self.h, abi.dptr(x0), nsteps, delta, abi.iptr(sd), bm_order,
+        rc = self.L.orc_vmc_replay(self.h, abi.dptr(x0), nsteps, delta, abi.iptr(sd), bm_order,
                                   abi.dptr(p), abi.dptr(age), abi.dptr(ltf), abi.dptr(e),
                                   abi.dptr(xf), abi.iptr(sf))
         if rc:

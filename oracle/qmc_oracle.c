@@ -10,66 +10,77 @@
  */
 #include "qmc_oracle.h"
 
-#include <complex.h>
-#include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <tgmath.h>
 
-typedef double complex zc;
+/* Arithmetic type of the evaluation.  The oracle is built twice: in double
+ * (libqmc_oracle.so, the reference's own precision and loop order) and in
+ * long double (libqmc_oracle_ld.so, -DORC_REAL="long double"): the same
+ * sums with a 64-bit significand, which serve as the exact value of the
+ * reference's formulas for the double inputs (tables, g-vectors,
+ * positions) when the rounding error of an FP64 result is measured.  The
+ * heap sort of the twist list and the random-number generator stay in
+ * double and integers in both builds. */
+#ifndef ORC_REAL
+#define ORC_REAL double
+#endif
+typedef ORC_REAL real;
+typedef ORC_REAL _Complex zc;
 
 struct orc {
   int ntypes, np[2], ipf[2], ipl[2], nptot, npnorm;
-  double hbs2m[2], el[2], eli[2], drt, drti, drti2;
+  real hbs2m[2], el[2], eli[2], drt, drti, drti2;
   int ntab, ldut;
-  double* ut;
+  real* ut;
   int32_t *ngrid, *iexp, *nk_ewald;
-  double* tail;
+  real* tail;
   int iu2[2][2], iu3[2][2], iub[2][2], iv2[2][2];
-  double v2value[2][2], v0;
+  real v2value[2][2], v0;
   int nk;
-  double* kvec;
+  real* kvec;
   int irhok[2], nrhok, ntheta;
-  double adrift, delta;
+  real adrift, delta;
   int pp_dist[2][2];
   int ng;
-  double* gvec; /* [ng][2] */
+  real* gvec; /* [ng][2] */
   qmc_info info;
   /* pair tables (distances, reptation.f:4127-4178) */
   int npairs;
-  double *pp_r, *pp_byr, *pp_rvec, *pp_rem;
+  real *pp_r, *pp_byr, *pp_rvec, *pp_rem;
   int* pp_ind;
-  double* rhok; /* [2][2*nk], column = type (reptation.f:4224-4243) */
-  double var[2], vari[2];
+  real* rhok; /* [2][2*nk], column = type (reptation.f:4224-4243) */
+  real var[2], vari[2];
 };
 
 #define UT(o, t, i, c) ((o)->ut[((size_t)(t)*4 + (c)) * (o)->ldut + (i)])
 
 /* ------------------------------------------------------------------------ */
 /* getf / getf_0: cubic spline on the uniform grid (reptation.f:4103-4125). */
-static void getf(const orc* o, int tab, int ind, double rem, double* f,
-                 double* df, double* ddf) {
+static void getf(const orc* o, int tab, int ind, real rem, real* f,
+                 real* df, real* ddf) {
   int t = tab - 1;
   int i = ind < o->ngrid[t] ? ind : o->ngrid[t];
-  double rm = rem * o->drti;
-  double t1 = UT(o, t, i, 0), t2 = UT(o, t, i, 1), t3 = UT(o, t, i, 2),
+  real rm = rem * o->drti;
+  real t1 = UT(o, t, i, 0), t2 = UT(o, t, i, 1), t3 = UT(o, t, i, 2),
          t4 = UT(o, t, i, 3);
   *f = t1 + rm * (t2 + rm * (t3 + rm * t4));
   *df = (t2 + rm * (2 * t3 + rm * (3 * t4))) * o->drti;
   *ddf = (t3 + rm * (3 * t4)) * o->drti2;
 }
 
-static double ipow(double b, int e) {
-  double r = 1.0;
+static real ipow(real b, int e) {
+  real r = 1.0;
   int n = e < 0 ? -e : e;
   for (int k = 0; k < n; k++) r *= b;
   return e < 0 ? 1.0 / r : r;
 }
 
-static double getf_0(const orc* o, int tab, int ind, double rem, double byr) {
+static real getf_0(const orc* o, int tab, int ind, real rem, real byr) {
   int t = tab - 1;
   int i = ind < o->ngrid[t] ? ind : o->ngrid[t];
-  double rm = rem * o->drti;
-  double f = UT(o, t, i, 0) +
+  real rm = rem * o->drti;
+  real f = UT(o, t, i, 0) +
              rm * (UT(o, t, i, 1) + rm * (UT(o, t, i, 2) + rm * UT(o, t, i, 3)));
   return f * ipow(byr, o->iexp[t]);
 }
@@ -106,16 +117,16 @@ orc* orc_create(const qmc_tables* t, int* err) {
   o->drti2 = 2.0 * o->drti * o->drti;
   o->ntab = t->ntab;
   o->ldut = t->ldut;
-  o->ut = (double*)malloc(sizeof(double) * 4 * (size_t)t->ldut * t->ntab);
-  memcpy(o->ut, t->ut, sizeof(double) * 4 * (size_t)t->ldut * t->ntab);
+  o->ut = (real*)malloc(sizeof(real) * 4 * (size_t)t->ldut * t->ntab);
+  for (size_t k = 0; k < 4 * (size_t)t->ldut * t->ntab; k++) o->ut[k] = t->ut[k];
   o->ngrid = (int32_t*)malloc(sizeof(int32_t) * t->ntab);
   o->iexp = (int32_t*)malloc(sizeof(int32_t) * t->ntab);
   o->nk_ewald = (int32_t*)malloc(sizeof(int32_t) * t->ntab);
-  o->tail = (double*)malloc(sizeof(double) * t->ntab);
+  o->tail = (real*)malloc(sizeof(real) * t->ntab);
   memcpy(o->ngrid, t->ngrid, sizeof(int32_t) * t->ntab);
   memcpy(o->iexp, t->iexp, sizeof(int32_t) * t->ntab);
   memcpy(o->nk_ewald, t->nk_ewald, sizeof(int32_t) * t->ntab);
-  memcpy(o->tail, t->tail, sizeof(double) * t->ntab);
+  for (int k = 0; k < t->ntab; k++) o->tail[k] = t->tail[k];
   for (int it = 0; it < 2; it++)
     for (int jt = 0; jt < 2; jt++) {
       int k = it * 2 + jt;
@@ -128,8 +139,8 @@ orc* orc_create(const qmc_tables* t, int* err) {
     }
   o->v0 = t->v0;
   o->nk = t->nk;
-  o->kvec = (double*)malloc(sizeof(double) * 2 * (t->nk > 0 ? t->nk : 1));
-  if (t->nk > 0) memcpy(o->kvec, t->kvec, sizeof(double) * 2 * t->nk);
+  o->kvec = (real*)malloc(sizeof(real) * 2 * (t->nk > 0 ? t->nk : 1));
+  for (int k = 0; k < 2 * t->nk; k++) o->kvec[k] = t->kvec[k];
   o->nrhok = t->nrhok;
   for (int it = 0; it < 2; it++) {
     o->irhok[it] = t->irhok[it];
@@ -153,12 +164,12 @@ orc* orc_create(const qmc_tables* t, int* err) {
         }
     }
   int np2 = o->nptot * (o->nptot + 1) / 2 + 1;
-  o->pp_r = (double*)calloc(np2, sizeof(double));
-  o->pp_byr = (double*)calloc(np2, sizeof(double));
-  o->pp_rem = (double*)calloc(np2, sizeof(double));
-  o->pp_rvec = (double*)calloc(2 * np2, sizeof(double));
+  o->pp_r = (real*)calloc(np2, sizeof(real));
+  o->pp_byr = (real*)calloc(np2, sizeof(real));
+  o->pp_rem = (real*)calloc(np2, sizeof(real));
+  o->pp_rvec = (real*)calloc(2 * np2, sizeof(real));
   o->pp_ind = (int*)calloc(np2, sizeof(int));
-  o->rhok = (double*)calloc(2 * 2 * (size_t)(o->nk > 0 ? o->nk : 1), sizeof(double));
+  o->rhok = (real*)calloc(2 * 2 * (size_t)(o->nk > 0 ? o->nk : 1), sizeof(real));
 
   /* property slots (reptation.f:577-635) */
   qmc_info* in = &o->info;
@@ -200,8 +211,8 @@ int orc_info(const orc* o, qmc_info* out) {
 
 int orc_set_twist(orc* o, const double* gvec, int ng) {
   free(o->gvec);
-  o->gvec = (double*)malloc(sizeof(double) * 2 * (ng > 0 ? ng : 1));
-  if (ng > 0) memcpy(o->gvec, gvec, sizeof(double) * 2 * ng);
+  o->gvec = (real*)malloc(sizeof(real) * 2 * (ng > 0 ? ng : 1));
+  for (int k = 0; k < 2 * ng; k++) o->gvec[k] = gvec[k];
   o->ng = ng;
   return QMC_OK;
 }
@@ -318,7 +329,7 @@ int orc_twist_gvec(const double* kvec, int nk, const double* el, int ntheta,
 
 /* ------------------------------------------------------------------------ */
 /* distances (reptation.f:4127-4178) and rhok (reptation.f:4224-4243).      */
-static void distances(orc* o, const double* x) {
+static void distances(orc* o, const real* x) {
   int ij = 0;
   for (int it = 0; it < o->ntypes; it++) {
     for (int jt = it; jt < o->ntypes; jt++) {
@@ -326,14 +337,14 @@ static void distances(orc* o, const double* x) {
       for (int ip = o->ipf[it]; ip <= o->ipl[it]; ip++) {
         int j0 = (jt == it) ? ip + 1 : o->ipf[jt];
         for (int jp = j0; jp <= o->ipl[jt]; jp++) {
-          double r2 = 0.0;
+          real r2 = 0.0;
           for (int d = 0; d < 2; d++) {
-            double v = x[2 * ip + d] - x[2 * jp + d];
+            real v = x[2 * ip + d] - x[2 * jp + d];
             v = v - o->el[d] * round(o->eli[d] * v);
             o->pp_rvec[2 * ij + d] = v;
             r2 = r2 + v * v;
           }
-          double r = sqrt(r2);
+          real r = sqrt(r2);
           o->pp_r[ij] = r;
           o->pp_ind[ij] = (int)(r * o->drti);
           o->pp_rem[ij] = r - o->pp_ind[ij] * o->drt;
@@ -347,11 +358,11 @@ static void distances(orc* o, const double* x) {
   if (o->nrhok != 0) {
     for (int it = 0; it < o->ntypes; it++) {
       if (!o->irhok[it]) continue;
-      double* rk = o->rhok + (size_t)it * 2 * o->nk;
+      real* rk = o->rhok + (size_t)it * 2 * o->nk;
       for (int k = 0; k < 2 * o->nk; k++) rk[k] = 0.0;
       for (int ip = o->ipf[it]; ip <= o->ipl[it]; ip++)
         for (int ik = 0; ik < o->nk; ik++) {
-          double kr = 0.0;
+          real kr = 0.0;
           for (int d = 0; d < 2; d++) kr = kr + o->kvec[2 * ik + d] * x[2 * ip + d];
           rk[2 * ik] += cos(kr);
           rk[2 * ik + 1] += sin(kr);
@@ -382,19 +393,19 @@ static void distances(orc* o, const double* x) {
   } while (0)
 
 typedef struct wf {
-  double ltf;
-  double* g; /* [nptot][2] */
-  double h[2];
-  double grad2_ph;
+  real ltf;
+  real* g; /* [nptot][2] */
+  real h[2];
+  real grad2_ph;
   int guard;
 } wf;
 
 /* two_body (reptation.f:2981-3046) */
 static void two_body(orc* o, wf* w) {
-  double ltf = 0.0, h[2] = {0, 0};
-  double* g = (double*)calloc(2 * (size_t)o->nptot, sizeof(double));
+  real ltf = 0.0, h[2] = {0, 0};
+  real* g = (real*)calloc(2 * (size_t)o->nptot, sizeof(real));
   FOR_PAIRS(o, iu2, {
-    double t, dt, ddt;
+    real t, dt, ddt;
     getf(o, tab, o->pp_ind[ij], o->pp_rem[ij], &t, &dt, &ddt);
     ltf = ltf + t;
     dt = dt * o->pp_byr[ij];
@@ -422,19 +433,19 @@ static void three_body(orc* o, wf* w) {
     for (int jt = it; jt < o->ntypes; jt++) any += o->iu3[it][jt] != 0;
   if (!any) return;
   int n = o->nptot;
-  double* G = (double*)calloc(2 * (size_t)n, sizeof(double));
-  double* dg = (double*)calloc(4 * (size_t)n * n, sizeof(double));
-  double* ddg = (double*)calloc(2 * (size_t)n * n, sizeof(double));
+  real* G = (real*)calloc(2 * (size_t)n, sizeof(real));
+  real* dg = (real*)calloc(4 * (size_t)n * n, sizeof(real));
+  real* ddg = (real*)calloc(2 * (size_t)n * n, sizeof(real));
 #define DG(jd, id, jp_, ip_) dg[((((size_t)(ip_)) * n + (jp_)) * 2 + (id)) * 2 + (jd)]
 #define DDG(id, jp_, ip_) ddg[(((size_t)(ip_)) * n + (jp_)) * 2 + (id)]
   FOR_PAIRS(o, iu3, {
-    double t, dt, ddt;
+    real t, dt, ddt;
     getf(o, tab, o->pp_ind[ij], o->pp_rem[ij], &t, &dt, &ddt);
-    const double* delx = o->pp_rvec + 2 * ij;
-    double rij = o->pp_byr[ij];
+    const real* delx = o->pp_rvec + 2 * ij;
+    real rij = o->pp_byr[ij];
     /* makeg (reptation.f:2884-2926) */
     {
-      double aux1 = t, aux2, aux3;
+      real aux1 = t, aux2, aux3;
       for (int d = 0; d < 2; d++) {
         aux2 = -aux1 * delx[d];
         G[2 * ip + d] += aux2;
@@ -462,7 +473,7 @@ static void three_body(orc* o, wf* w) {
       }
     }
     /* remove a two-body term (reptation.f:2822-2832) */
-    double r = o->pp_r[ij];
+    real r = o->pp_r[ij];
     ddt = 2 * (t * t + 4 * r * t * dt + r * r * (dt * dt + ddt * t));
     dt = 2 * t * (t + r * dt);
     t = (t * r) * (t * r);
@@ -481,7 +492,7 @@ static void three_body(orc* o, wf* w) {
   /* dotg (reptation.f:2928-2979) */
   for (int ip = 0; ip < n; ip++)
     for (int d = 0; d < 2; d++) w->ltf -= 0.5 * G[2 * ip + d] * G[2 * ip + d];
-  double* gn = w->g;
+  real* gn = w->g;
   for (int it = 0; it < o->ntypes; it++) {
     if (o->iu3[it][it]) {
       for (int ip = o->ipf[it]; ip <= o->ipl[it]; ip++) {
@@ -543,14 +554,14 @@ static void three_body(orc* o, wf* w) {
 /* ------------------------------------------------------------------------ */
 /* LINPACK zgefa / zgedi(job=11) restated (reptation.f:9227-9472);
  * column-major a(row, col) = a[col*n + row]. */
-static double cabs1(zc z) { return fabs(creal(z)) + fabs(cimag(z)); }
+static real cabs1(zc z) { return fabs(creal(z)) + fabs(cimag(z)); }
 
 static int zgefa(zc* a, int n, int* ipvt) {
   int info = 0;
 #define A(r, c) a[(size_t)(c) * n + (r)]
   for (int k = 0; k < n - 1; k++) {
     int l = k;
-    double smax = cabs1(A(k, k));
+    real smax = cabs1(A(k, k));
     for (int i = k + 1; i < n; i++)
       if (cabs1(A(i, k)) > smax) { l = i; smax = cabs1(A(i, k)); }
     ipvt[k] = l;
@@ -608,14 +619,14 @@ static void zgedi(zc* a, int n, const int* ipvt, zc det[2], zc* work) {
 }
 
 /* slater_excitations + paho_orbitals (reptation.f:9967-10363) */
-static int slater(orc* o, const double* x, wf* w) {
+static int slater(orc* o, const real* x, wf* w) {
   if (!o->info.has_det) return QMC_OK;
   int n = o->nptot;
   for (int it = 0; it < o->ntypes; it++)
     if (o->np[it] > o->ng) return QMC_ERR_NOTWIST;
-  double* qx = (double*)malloc(sizeof(double) * 2 * n);
-  double* qa = (double*)calloc(4 * (size_t)n * n, sizeof(double));
-  double* qb = (double*)calloc(2 * (size_t)n * n, sizeof(double));
+  real* qx = (real*)malloc(sizeof(real) * 2 * n);
+  real* qa = (real*)calloc(4 * (size_t)n * n, sizeof(real));
+  real* qb = (real*)calloc(2 * (size_t)n * n, sizeof(real));
 #define QA(jd, id, jp_, ip_) qa[((((size_t)(ip_)) * n + (jp_)) * 2 + (id)) * 2 + (jd)]
 #define QB(id, jp_, ip_) qb[(((size_t)(ip_)) * n + (jp_)) * 2 + (id)]
   for (int ip = 0; ip < n; ip++)
@@ -624,10 +635,10 @@ static int slater(orc* o, const double* x, wf* w) {
       QA(d, d, ip, ip) = 1.0;
     }
   FOR_PAIRS(o, iub, {
-    double t, dt, ddt;
+    real t, dt, ddt;
     getf(o, tab, o->pp_ind[ij], o->pp_rem[ij], &t, &dt, &ddt);
     for (int d = 0; d < 2; d++) {
-      double v = t * o->pp_rvec[2 * ij + d];
+      real v = t * o->pp_rvec[2 * ij + d];
       qx[2 * ip + d] += v;
       qx[2 * jp + d] -= v;
     }
@@ -643,9 +654,9 @@ static int slater(orc* o, const double* x, wf* w) {
     dA[jt] = (zc*)malloc(sizeof(zc) * 2 * mm * mm);
     ddA[jt] = (zc*)malloc(sizeof(zc) * 4 * mm * mm);
     for (int jk = 0; jk < m; jk++) {
-      const double* gk = o->gvec + 2 * jk;
+      const real* gk = o->gvec + 2 * jk;
       for (int jp = o->ipf[jt]; jp <= o->ipl[jt]; jp++) {
-        double kr = 0.0;
+        real kr = 0.0;
         for (int d = 0; d < 2; d++) kr = kr + qx[2 * jp + d] * gk[d];
         int i = jp - o->ipf[jt];
         zc z = cos(kr) + I * sin(kr);
@@ -667,10 +678,10 @@ static int slater(orc* o, const double* x, wf* w) {
     }
     zc det[2];
     zgedi(inv[jt], m, ipvt, det, wk);
-    w->ltf -= creal(clog(det[0]) + det[1] * log(10.0));
+    w->ltf -= creal(log(det[0]) + det[1] * log((real)10));
     free(ipvt); free(wk);
   }
-  double ww = exp(-w->ltf * 2);
+  real ww = exp(-w->ltf * 2);
   if (ww <= 0.0) {
     w->guard = 1;
     for (int k = 0; k < o->ntypes; k++) { free(inv[k]); free(dA[k]); free(ddA[k]); }
@@ -678,12 +689,12 @@ static int slater(orc* o, const double* x, wf* w) {
     return QMC_OK;
   }
   FOR_PAIRS(o, iub, {
-    double t, dt, ddt;
+    real t, dt, ddt;
     getf(o, tab, o->pp_ind[ij], o->pp_rem[ij], &t, &dt, &ddt);
-    const double* rv = o->pp_rvec + 2 * ij;
-    double byr = o->pp_byr[ij];
+    const real* rv = o->pp_rvec + 2 * ij;
+    real byr = o->pp_byr[ij];
     for (int id = 0; id < 2; id++) {
-      double w2 = (dt * byr * (2 + 1) + ddt) * rv[id];
+      real w2 = (dt * byr * (2 + 1) + ddt) * rv[id];
       QB(id, ip, ip) += w2;
       QB(id, jp, jp) -= w2;
       QB(id, jp, ip) += w2;
@@ -693,7 +704,7 @@ static int slater(orc* o, const double* x, wf* w) {
       QA(id, id, jp, ip) -= t;
       QA(id, id, ip, jp) -= t;
       for (int jd = 0; jd < 2; jd++) {
-        double w3 = dt * byr * rv[jd] * rv[id];
+        real w3 = dt * byr * rv[jd] * rv[id];
         QA(jd, id, ip, ip) += w3;
         QA(jd, id, jp, jp) += w3;
         QA(jd, id, jp, ip) -= w3;
@@ -701,7 +712,7 @@ static int slater(orc* o, const double* x, wf* w) {
       }
     }
   });
-  double aux = 0.0;
+  real aux = 0.0;
   for (int jt = 0; jt < o->ntypes; jt++) {
     int m = o->np[jt];
     if (m == 0) continue;
@@ -719,7 +730,7 @@ static int slater(orc* o, const double* x, wf* w) {
         }
     for (int it = 0; it < o->ntypes; it++) {
       if (!o->iub[it][jt]) continue;
-      double aux2 = 0.0;
+      real aux2 = 0.0;
       for (int ip = o->ipf[it]; ip <= o->ipl[it]; ip++)
         for (int id = 0; id < 2; id++) {
           zc z = 0.0;
@@ -744,7 +755,7 @@ static int slater(orc* o, const double* x, wf* w) {
               for (int kf = 0; kf < m; kf++)
                 z1 = z1 + Ai[(size_t)kf * m + jf] *
                               ddA[jt][(((size_t)(id * 2 + jd)) * m + jf) * m + kf];
-              double z2 = 0.0;
+              real z2 = 0.0;
               for (int kd = 0; kd < 2; kd++) z2 = z2 + QA(kd, jd, jp, ip) * QA(kd, id, jp, ip);
               w->h[it] -= creal(z1) * z2;
             }
@@ -777,25 +788,25 @@ static int slater(orc* o, const double* x, wf* w) {
 
 /* ------------------------------------------------------------------------ */
 /* kinetic (reptation.f:2729-2775) */
-static void kinetic(orc* o, wf* w, double* p) {
+static void kinetic(orc* o, wf* w, real* p) {
   const qmc_info* in = &o->info;
   int n_jgg = 0;
   p[in->jkin0 - 1] = 0.0;
   p[in->jgg - 1] = 0.0;
   for (int it = 0; it < o->ntypes; it++) {
     if (o->hbs2m[it] == 0.0) continue;
-    double g2 = 0.0;
+    real g2 = 0.0;
     if (o->adrift == 0.0) {
       for (int ip = o->ipf[it]; ip <= o->ipl[it]; ip++)
         for (int d = 0; d < 2; d++) g2 = g2 + w->g[2 * ip + d] * w->g[2 * ip + d];
       p[in->jgg - 1] = 1.0;
       n_jgg = 1;
     } else {
-      double vari = 1.0 / (2.0 * o->hbs2m[it] * o->delta);
+      real vari = 1.0 / (2.0 * o->hbs2m[it] * o->delta);
       for (int ip = o->ipf[it]; ip <= o->ipl[it]; ip++) {
-        double x2 = 0.0;
+        real x2 = 0.0;
         for (int d = 0; d < 2; d++) x2 = x2 + w->g[2 * ip + d] * w->g[2 * ip + d];
-        double gg = x2 * o->adrift * o->delta * o->delta * vari;
+        real gg = x2 * o->adrift * o->delta * o->delta * vari;
         gg = (-1.0 + sqrt(1.0 + 2.0 * gg)) / gg;
         p[in->jgg - 1] += gg;
         n_jgg++;
@@ -803,7 +814,7 @@ static void kinetic(orc* o, wf* w, double* p) {
         g2 = g2 + x2;
       }
     }
-    double kin = o->hbs2m[it] * (w->h[it] - g2);
+    real kin = o->hbs2m[it] * (w->h[it] - g2);
     p[in->jkin0 - 1] += kin;
     p[in->jkin[it] - 1] = kin / o->np[it];
     p[in->jun[it] - 1] = 1.0 / sqrt(g2);
@@ -816,21 +827,21 @@ static void kinetic(orc* o, wf* w, double* p) {
 
 /* potential (reptation.f:3986-4101), egas subset: pair potential with tails,
  * k-space Ewald through rhok, v0. */
-static void potential(orc* o, double* p) {
+static void potential(orc* o, real* p) {
   const qmc_info* in = &o->info;
-  double p2[3] = {0, 0, 0}; /* [0]=total, [1+it] per type */
+  real p2[3] = {0, 0, 0}; /* [0]=total, [1+it] per type */
   int ij = 0;
   for (int it = 0; it < o->ntypes; it++) {
     for (int jt = it; jt < o->ntypes; jt++) {
       int tab = o->iv2[it][jt];
       if (!tab) continue;
-      double v = o->v2value[it][jt];
+      real v = o->v2value[it][jt];
       if (jt == it) {
-        double aux = o->np[it] * (o->np[it] - 1) * 0.5 * o->tail[tab - 1] * v;
+        real aux = o->np[it] * (o->np[it] - 1) * 0.5 * o->tail[tab - 1] * v;
         p2[1 + it] += aux;
         p2[0] += aux;
       } else {
-        double aux = o->np[it] * o->np[jt] * o->tail[tab - 1] * v;
+        real aux = o->np[it] * o->np[jt] * o->tail[tab - 1] * v;
         p2[1 + it] += 0.5 * aux;
         p2[1 + jt] += 0.5 * aux;
         p2[0] += aux;
@@ -838,7 +849,7 @@ static void potential(orc* o, double* p) {
       for (int ip = o->ipf[it]; ip <= o->ipl[it]; ip++) {
         int j0 = (jt == it) ? ip + 1 : o->ipf[jt];
         for (int jp = j0; jp <= o->ipl[jt]; jp++) {
-          double f = getf_0(o, tab, o->pp_ind[ij], o->pp_rem[ij], o->pp_byr[ij]);
+          real f = getf_0(o, tab, o->pp_ind[ij], o->pp_rem[ij], o->pp_byr[ij]);
           if (jt == it) {
             p2[1 + it] += f * v;
             p2[0] += f * v;
@@ -857,13 +868,13 @@ static void potential(orc* o, double* p) {
       int tab = o->iv2[it][jt];
       if (!tab || o->nk_ewald[tab - 1] == 0) continue;
       int t = tab - 1, ng = o->ngrid[t], nke = o->nk_ewald[t];
-      const double* ri = o->rhok + (size_t)it * 2 * o->nk;
-      const double* rj = o->rhok + (size_t)jt * 2 * o->nk;
-      double aux = 0.0;
+      const real* ri = o->rhok + (size_t)it * 2 * o->nk;
+      const real* rj = o->rhok + (size_t)jt * 2 * o->nk;
+      real aux = 0.0;
       for (int ik = 0; ik < nke; ik++)
         aux = aux + UT(o, t, ik + 1 + ng, 0) *
                         (ri[2 * ik] * rj[2 * ik] + ri[2 * ik + 1] * rj[2 * ik + 1]);
-      double vk0 = UT(o, t, ng + nke + 1, 0);
+      real vk0 = UT(o, t, ng + nke + 1, 0);
       p2[1 + it] += aux + vk0 * o->npnorm;
       p2[0] += aux + vk0 * o->npnorm;
     }
@@ -873,8 +884,8 @@ static void potential(orc* o, double* p) {
 }
 
 /* compute_properties(1) (reptation.f:2661-2727) */
-int orc_eval(orc* o, const double* x, double* props, double* g, double* h,
-             double* grad2_ph, int32_t* status) {
+static int eval_real(orc* o, const real* x, real* props, real* g, real* h,
+                     real* grad2_ph, int32_t* status) {
   const qmc_info* in = &o->info;
   distances(o, x);
   wf w;
@@ -887,8 +898,8 @@ int orc_eval(orc* o, const double* x, double* props, double* g, double* h,
   three_body(o, &w);
   int rc = slater(o, x, &w);
   if (rc != QMC_OK) return rc;
-  double jacc_saved = props[in->jacc - 1];
-  double* p = props;
+  real jacc_saved = props[in->jacc - 1];
+  real* p = props;
   p[in->jltf - 1] = w.ltf;
   for (int it = 0; it < o->ntypes; it++) h[it] = w.h[it];
   kinetic(o, &w, p);
@@ -900,7 +911,7 @@ int orc_eval(orc* o, const double* x, double* props, double* g, double* h,
   int i = in->jrhok - 1;
   for (int it = 0; it < o->ntypes; it++) {
     if (!o->irhok[it]) continue;
-    const double* rk = o->rhok + (size_t)it * 2 * o->nk;
+    const real* rk = o->rhok + (size_t)it * 2 * o->nk;
     for (int ik = 0; ik < o->nk; ik++) {
       p[i] = rk[2 * ik];
       p[i + 1] = rk[2 * ik + 1];
@@ -910,6 +921,25 @@ int orc_eval(orc* o, const double* x, double* props, double* g, double* h,
   if (grad2_ph) *grad2_ph = w.grad2_ph;
   if (status) *status = w.guard;
   return QMC_OK;
+}
+
+int orc_eval(orc* o, const double* x, double* props, double* g, double* h,
+             double* grad2_ph, int32_t* status) {
+  const int n2 = 2 * o->nptot, np_ = o->info.n_props;
+  real* xr = (real*)malloc(sizeof(real) * (n2 + np_ + n2 + 2));
+  real *pr = xr + n2, *gr = pr + np_, *hr = gr + n2, g2r = grad2_ph ? *grad2_ph : 0.0;
+  for (int i = 0; i < n2; i++) xr[i] = x[i];
+  for (int i = 0; i < np_; i++) pr[i] = props[i];
+  hr[0] = h[0]; hr[1] = h[1];
+  int rc = eval_real(o, xr, pr, gr, hr, grad2_ph ? &g2r : NULL, status);
+  if (rc == QMC_OK) {
+    for (int i = 0; i < np_; i++) props[i] = (double)pr[i];
+    for (int i = 0; i < n2; i++) g[i] = (double)gr[i];
+    for (int it = 0; it < o->ntypes; it++) h[it] = (double)hr[it];
+    if (grad2_ph) *grad2_ph = (double)g2r;
+  }
+  free(xr);
+  return rc;
 }
 
 int orc_eval_batch(orc* o, int nw, const double* x, double* props, double* g,
@@ -956,6 +986,12 @@ double orc_rannyu(orc_rng* r) {
   return twom12 * (l1 + twom12 * (l2 + twom12 * (l3 + twom12 * l4)));
 }
 
+/* the COMMON delta that kinetic's drift cutoff reads (reptation.f:2755) */
+int orc_set_delta(orc* o, double delta) {
+  o->delta = delta;
+  return 0;
+}
+
 /* move + metropolis_test replay (reptation.f:4617-4680, 2622-2659), with
  * read_conf's start (reptation.f:2486-2495). */
 int orc_vmc_replay(orc* o, const double* x0, int nsteps, double delta,
@@ -972,6 +1008,8 @@ int orc_vmc_replay(orc* o, const double* x0, int nsteps, double delta,
   double* po = (double*)calloc(np_, sizeof(double));
   double* pn = (double*)calloc(np_, sizeof(double));
   double ho[2], hn[2], g2ph = 0.0;
+  /* the vmc record sets delta, then sigma (reptation.f:740-746, 855-870) */
+  o->delta = delta;
   for (int it = 0; it < o->ntypes; it++) {
     o->var[it] = 2.0 * o->hbs2m[it] * delta;
     o->vari[it] = 1.0 / o->var[it];

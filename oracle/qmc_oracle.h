@@ -29,6 +29,10 @@ int orc_set_twist(orc* o, const double* gvec, int ng);
 int orc_twist_gvec(const double* kvec, int nk, const double* el, int ntheta,
                    int ith, int jth, double* gvec_out, int variant);
 
+/* Current time step read by the drift cutoff of kinetic when adrift != 0
+ * (the COMMON delta, reptation.f:2755). */
+int orc_set_delta(orc* o, double delta);
+
 /* compute_properties(1) for one walker (reptation.f:2661-2727). */
 int orc_eval(orc* o, const double* x, double* props, double* g, double* h,
              double* grad2_ph, int32_t* status);

@@ -11,7 +11,9 @@ c                        3 = timing: mode 1 without output, repeated
 c                            ncfg times over the first config block
 c   ith jth              twist indices (used when the deck sets ntwists)
 c   ncfg                 configurations (mode 1) or moves (mode 2)
-c   delta                time step of the move (mode 2)
+c   delta                time step of the move (mode 2, 4); in mode 1
+c                        a positive value sets the COMMON delta (sigma)
+c                        read by the drift cutoff of decks with adrift
 c   cfgfile              raw float64 positions, [ncfg][nptot][2]
 c   outfile              raw output stream (layout below)
 c
@@ -66,6 +68,10 @@ c the g-vector list in use (twistg or input's untwisted list)
      &    ,status='replace')
       write(13) n_scalar_props,nptot,ntypes,nk_out,nrhok
       if(mode.eq.1)then
+       if(del.gt.0.d0)then
+        delta=del
+        call sigma
+       endif
        do icfg=1,ncfg
         read(12) ((x_new(idim,ip),idim=1,2),ip=1,nptot)
         call compute_properties(1)

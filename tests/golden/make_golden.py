@@ -14,9 +14,15 @@ live under tests/golden/decks/:
   g61p     generated polarized N=61 (+ an empty second type), rs=10
   n26r     n26 at the Gamma point without the rhok properties (reptation
            fixtures: 14 property slots per bead)
+  n26a     n26 at the Gamma point with the drift cutoff on (adrift 0.5)
+  n26r_gs, n26r_gsn, n26r_wp, n26r_rj0, n26r_ec
+           n26r with the propagator keywords of reptation_move: gstorto,
+           gstorto + nodalaction, wpiu, rejection 0, ecut with a value (with
+           the drift cutoff on, without which ecut changes nothing)
 
-Usage: python tests/golden/make_golden.py [rmc]   (needs oracle/_ref; minutes;
-       "rmc" writes only the reptation fixtures)
+Usage: python tests/golden/make_golden.py [rmc|rmcvar|adrift]   (needs
+       oracle/_ref; minutes; "rmc" writes only the reptation fixtures,
+       "rmcvar" the propagator variants, "adrift" the drift-cutoff ones)
 """
 import importlib
 import os
@@ -37,9 +43,20 @@ DECKS = os.path.join(HERE, "decks")
 WORK = "/tmp/degas_golden"
 
 
+# n26r variants: extra deck lines selecting reptation_move's propagator
+# options (reptation.f:149-170, 1904-1919, 1973-2101)
+RMC_VARIANTS = {
+    "n26r_gs": ["gstorto 1"],
+    "n26r_gsn": ["gstorto 1", "nodalaction 1"],
+    "n26r_wp": ["wpiu 1"],
+    "n26r_rj0": ["rejection 0"],
+    "n26r_ec": ["ecut 1 -0.4", "adrift 0.5"],
+}
+
+
 def derive_n26_variants():
     src = os.path.join(DECKS, "n26")
-    for name in ("n26g", "n26p", "free26", "n26r"):
+    for name in ("n26g", "n26p", "free26", "n26r", "n26a") + tuple(RMC_VARIANTS):
         dst = os.path.join(DECKS, name)
         shutil.rmtree(dst, ignore_errors=True)
         os.makedirs(dst)
@@ -55,8 +72,12 @@ def derive_n26_variants():
                 continue
             if w[0] == "ntwists":
                 continue
-            if name == "n26r" and w[0] == "rhok":
+            if name.startswith("n26r") and w[0] == "rhok":
                 continue
+            if w[0] == "end":
+                if name == "n26a":
+                    out.append("adrift 0.5")
+                out.extend(RMC_VARIANTS.get(name, []))
             if name == "free26" and w[0] in ("u2", "u3"):
                 continue
             if name in ("n26p", "free26") and w[0] == "backflow":
@@ -76,26 +97,28 @@ def make_generated():
     deckgen.make_deck(os.path.join(DECKS, "g61p"), "g61p", 61, 0, 10.0, 200)
 
 
-def eval_fixture(name, nconf, seed, twist=(1, 1), extra=None):
+def eval_fixture(name, nconf, seed, twist=(1, 1), extra=None, delta=0.0):
     d = deck.read_deck(os.path.join(DECKS, name))
     x = deckgen.random_configs(nconf, d.nptot, d.el[0], seed)
     if extra is not None:
         x = np.concatenate([extra, x])
-    raw, _ = po.run_refeval(os.path.join(DECKS, name), os.path.join(WORK, name), 1, x, twist=twist)
+    raw, _ = po.run_refeval(os.path.join(DECKS, name), os.path.join(WORK, name), 1, x, twist=twist,
+                            delta=delta)
     r = po.parse_eval(raw, d.nptot)
     out = os.path.join(HERE, f"eval_{name}_t{twist[0]}{twist[1]}.npz")
     np.savez_compressed(out, x=x, scalars=r["scalars"], g=r["g"], h=r["h"],
-                        grad2_ph=r["grad2_ph"], rhok=r["rhok"], twist=np.array(twist))
+                        grad2_ph=r["grad2_ph"], rhok=r["rhok"], twist=np.array(twist),
+                        delta=delta)
     print("wrote", out, x.shape)
 
 
-def replay_fixture(name, nsteps, every, delta, x0):
+def replay_fixture(name, nsteps, every, delta, x0, tag=None):
     d = deck.read_deck(os.path.join(DECKS, name))
     raw, _ = po.run_refeval(os.path.join(DECKS, name), os.path.join(WORK, name + "_r"), 2,
                             x0[None], twist=(1, 1), delta=delta, nsteps=nsteps)
     r = po.parse_replay(raw, d.nptot, nsteps)
     ck = np.arange(0, nsteps, every)
-    out = os.path.join(HERE, f"replay_{name}.npz")
+    out = os.path.join(HERE, f"replay_{tag or name}.npz")
     np.savez_compressed(out, delta=delta, every=every, ck_seeds=r["seeds"][ck],
                         ck_x=r["x_old"][ck], p=r["p"], age=r["age"], ltf_new=r["ltf_new"],
                         e_new=r["e_new"], first_seed=r["seeds"][0])
@@ -133,6 +156,7 @@ def make_rmc():
     rmc_fixture("n26r", 3000, 100, 0.05,
                 dict(ntau=16, ntauskip=3, mdelta=4, etrial=-0.45, mage=1e9, mage_r=1e9),
                 random_path(shipped[0], 16, 0.02, 31))
+    make_rmc_variants(shipped[0])
     L58 = deck.read_deck(os.path.join(DECKS, "g58")).el[0]
     g = (np.arange(8) + 0.5) / 8 * L58 - L58 / 2
     sites = np.array([(a, b) for a in g for b in g])
@@ -141,6 +165,26 @@ def make_rmc():
     rmc_fixture("g58", 500, 100, 1.0,
                 dict(ntau=10, ntauskip=2, mdelta=3, etrial=-0.3, mage=4, mage_r=6),
                 random_path(x58, 10, 0.1, 34))
+
+
+def make_rmc_variants(x0):
+    """One 600-move reptation record per propagator option of the deck."""
+    derive_n26_variants()
+    for k, name in enumerate(RMC_VARIANTS):
+        rmc_fixture(name, 600, 100, 0.01,
+                    dict(ntau=16, ntauskip=3, mdelta=4, etrial=-0.45, mage=1e9, mage_r=1e9),
+                    random_path(x0, 16, 0.005, 40 + k))
+
+
+def make_adrift():
+    """The drift cutoff: an evaluation at delta 0.05 and two VMC records at
+    different time steps on the same deck."""
+    derive_n26_variants()
+    shipped = deck.read_configs([os.path.join(DECKS, "n26", f) for f in
+                                 ("n26.up.x.res.0", "n26.down.x.res.0")], [13, 13])
+    eval_fixture("n26a", 16, 22, extra=shipped, delta=0.05)
+    replay_fixture("n26a", 1000, 100, 0.05, shipped[0], tag="n26a_d05")
+    replay_fixture("n26a", 1000, 100, 0.02, shipped[0], tag="n26a_d02")
 
 
 def main():
@@ -184,9 +228,16 @@ def main():
 
 
 if __name__ == "__main__":
+    os.makedirs(WORK, exist_ok=True)
     if sys.argv[1:] == ["rmc"]:
-        os.makedirs(WORK, exist_ok=True)
         make_rmc()
+    elif sys.argv[1:] == ["adrift"]:
+        make_adrift()
+    elif sys.argv[1:] == ["rmcvar"]:
+        shipped = deck.read_configs([os.path.join(DECKS, "n26", f) for f in
+                                     ("n26.up.x.res.0", "n26.down.x.res.0")], [13, 13])
+        make_rmc_variants(shipped[0])
     else:
         main()
         make_rmc()
+        make_adrift()

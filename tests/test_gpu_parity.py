@@ -15,7 +15,8 @@ pytestmark = pytest.mark.gpu
 RTOL = 1e-10
 EVAL_FIXTURES = ["eval_n26_t11.npz", "eval_n26_t23.npz", "eval_n26_t44.npz",
                  "eval_n26g_t11.npz", "eval_n26p_t11.npz", "eval_free26_t11.npz",
-                 "eval_g58_t11.npz", "eval_g122_t11.npz", "eval_g61p_t11.npz"]
+                 "eval_g58_t11.npz", "eval_g122_t11.npz", "eval_g61p_t11.npz",
+                 "eval_n26a_t11.npz"]
 
 _ctx_cache = {}
 
@@ -33,6 +34,8 @@ def test_hip_matches_reference_goldens(fn):
     print(f"\n[gpu] golden {fn}", flush=True)
     gold = load_golden(fn)
     c = ctx_for(fn.split("_")[1], tuple(gold["twist"]))
+    if "delta" in gold and float(gold["delta"]) > 0:
+        c.set_delta(float(gold["delta"]))  # the drift cutoff's time step
     out = c.eval(gold["x"])
     errs = compare_eval(out, gold, gold["scalars"].shape[1], RTOL, fn)
     print("   max err", max(errs.values()), flush=True)

@@ -90,12 +90,16 @@ def _replay(name):
     return gold, rep, tr, md
 
 
-@pytest.mark.parametrize("name", ["n26r", "g58"])
+@pytest.mark.parametrize("name", ["n26r", "g58", "n26r_gs", "n26r_gsn", "n26r_wp", "n26r_rj0",
+                                  "n26r_ec"])
 def test_rmc_move_sequence_matches_reference(name):
     """Teacher-forced replay: each 100-move segment starts from the
     reference's recorded reptile and RNG state; the device reproduces every
     direction, every number of beads, every Langevin and reptation decision,
-    the Langevin probabilities, the path weights and the path averages."""
+    the Langevin probabilities, the path weights and the path averages.
+    The n26r_* decks select the propagator options of reptation_move
+    (gstorto, gstorto + nodalaction, wpiu, rejection 0, ecut with the drift
+    cutoff on)."""
     print(f"\n[gpu] rmc replay {name}", flush=True)
     gold, rep, tr, md = _replay(name)
     n = len(gold["idir"])

@@ -8,16 +8,17 @@ from conftest import load_deck, load_golden, pkg
 pytestmark = pytest.mark.gpu
 
 
-def _replay(name):
+def _replay(name, tag=None, ctx=None):
     hip = pkg("hip")
     abi = pkg("abi")
-    gold = load_golden(f"replay_{name}.npz")
+    gold = load_golden(f"replay_{tag or name}.npz")
     d = load_deck(name)
-    ctx = hip.Context(d, twist=(1, 1))
+    if ctx is None:
+        ctx = hip.Context(d, twist=(1, 1))
     every = int(gold["every"])
     states = [abi.limbs_to_state(l) for l in gold["ck_seeds"]]
     wk = hip.Walkers(ctx, gold["ck_x"], states)
-    wk.start()
+    wk.start(delta=float(gold["delta"]))
     tp, ta = wk.sweep(every, float(gold["delta"]), trace=True)
     import torch
     torch.cuda.synchronize()
@@ -43,6 +44,23 @@ def test_vmc_accept_sequence_matches_reference(name):
     print(f"   moves {n} mismatches {mism} max rel p {relp:.2e} acc rate {acc.mean():.3f}", flush=True)
     assert mism == 0
     assert relp < 1e-8
+
+
+def test_vmc_adrift_two_time_steps_match_reference():
+    """The drift cutoff (adrift 0.5) reads the current time step: one
+    context replays the reference's VMC record at delta 0.05, then its
+    record at delta 0.02; both reproduce every decision and p."""
+    print("\n[gpu] vmc replay n26a at delta 0.05 then 0.02", flush=True)
+    hip = pkg("hip")
+    ctx = hip.Context(load_deck("n26a"), twist=(1, 1))
+    for tag in ("n26a_d05", "n26a_d02"):
+        gold, p, age = _replay("n26a", tag, ctx)
+        n = len(gold["p"])
+        mism = int(np.sum((age[:n] == 0) != (gold["age"][:n] == 0)))
+        relp = float(np.max(np.abs(p[:n] - gold["p"]) / np.maximum(gold["p"], 1e-300)))
+        print(f"   {tag}: moves {n} mismatches {mism} max rel p {relp:.2e}", flush=True)
+        assert mism == 0
+        assert relp < 1e-8
 
 
 def test_vmc_block_sums_accumulate_old_properties():

@@ -12,7 +12,8 @@ from conftest import compare_eval, load_deck, load_golden, pkg
 
 EVAL_FIXTURES = ["eval_n26_t11.npz", "eval_n26_t23.npz", "eval_n26_t44.npz",
                  "eval_n26g_t11.npz", "eval_n26p_t11.npz", "eval_free26_t11.npz",
-                 "eval_g58_t11.npz", "eval_g122_t11.npz", "eval_g61p_t11.npz"]
+                 "eval_g58_t11.npz", "eval_g122_t11.npz", "eval_g61p_t11.npz",
+                 "eval_n26a_t11.npz"]
 
 
 def _deck_of(fn):
@@ -24,6 +25,8 @@ def test_oracle_matches_reference(fn, oracle_mod):
     gold = load_golden(fn)
     d = load_deck(_deck_of(fn))
     o = oracle_mod.Oracle(d, twist=tuple(gold["twist"]))
+    if "delta" in gold and float(gold["delta"]) > 0:
+        o.set_delta(float(gold["delta"]))
     out = o.eval(gold["x"])
     compare_eval(out, gold, gold["scalars"].shape[1], 1e-10, fn)
 
@@ -134,3 +137,45 @@ def test_lu_logdet_matches_lapack(oracle_mod):
             A = np.exp(1j * (g @ q.T))
             tot -= np.linalg.slogdet(A)[1]
         assert abs(out["props"][w, 1] - tot) < 1e-10 * max(1.0, abs(tot))
+
+
+def test_drift_cutoff_is_active_and_finite(oracle_mod):
+    """The adrift deck (n26a) exercises kinetic's cutoff branch
+    (reptation.f:2747-2760): gg < 1 for every configuration, finite drift,
+    and the cut drift differs from the uncut one of the same deck read
+    without adrift (n26g)."""
+    gold = load_golden("eval_n26a_t11.npz")
+    jgg = 6
+    assert np.all(np.isfinite(gold["scalars"])) and np.all(np.isfinite(gold["g"]))
+    assert np.all(gold["scalars"][:, jgg] < 1.0)
+    o = oracle_mod.Oracle(load_deck("n26g"))
+    plain = o.eval(gold["x"])
+    assert np.all(np.abs(plain["g"]) >= np.abs(gold["g"]) - 1e-12)
+    assert np.max(np.abs(plain["g"] - gold["g"])) > 1e-6
+
+
+@pytest.mark.parametrize("tag", ["n26a_d05", "n26a_d02"])
+def test_oracle_replays_adrift_vmc(tag, oracle_mod):
+    """VMC with the drift cutoff at two time steps: the oracle's replay
+    from each 100-move checkpoint of the reference run reproduces its
+    accept/reject decisions (first 300 moves)."""
+    gold = load_golden(f"replay_{tag}.npz")
+    o = oracle_mod.Oracle(load_deck("n26a"))
+    every, delta = int(gold["every"]), float(gold["delta"])
+    mism = 0
+    for c in range(3):
+        r = o.vmc_replay(gold["ck_x"][c], every, delta, gold["ck_seeds"][c])
+        ref_age = gold["age"][c * every:(c + 1) * every]
+        mism += int(np.sum((r["age"] == 0) != (ref_age == 0)))
+        assert np.allclose(r["p"], gold["p"][c * every:(c + 1) * every], rtol=1e-8, atol=0)
+    assert mism == 0
+
+
+def test_rel_err_rejects_nan():
+    """A NaN anywhere fails a comparison unless the caller marks the slot
+    as an empty type's 0/0."""
+    from conftest import rel_err
+    assert rel_err([np.nan], [np.nan]) == float("inf")
+    assert rel_err([1.0], [np.nan]) == float("inf")
+    assert rel_err([np.nan], [np.nan], nan_ok=True) == 0.0
+    assert rel_err([1.0, np.nan], [1.0, 2.0], nan_ok=True) == float("inf")
