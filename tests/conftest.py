@@ -80,6 +80,71 @@ def compare_eval(out, gold, n_scalar, rtol, label=""):
     return errs
 
 
+def config_errors(out, ref, info, ntypes):
+    """Per-configuration relative errors of an evaluation `out` against a
+    reference evaluation `ref` (dicts with props [nw][n_props], g [nw][N][2],
+    h [nw][2]); returns quantity -> array over configurations.
+
+    Scales, per configuration: -log psi and the nodal distances against
+    their own magnitude; every energy slot against max(|value|, E_s) with
+    E_s = |epot| + |ekin| + |efp| (the terms the local energy adds, the scale
+    its rounding acts on); e2 against E_s^2; the drift as max_i |g_i -
+    g_i^ref| over max_i |g_i^ref|; the per-type laplacian h_t against
+    max(|h_t|, sum over the type of |g_i^ref|^2), the two terms whose
+    difference is the kinetic energy."""
+    p, q = np.asarray(out["props"]), np.asarray(ref["props"])
+    j = lambda k: info[k] - 1
+
+    def rel(a, b, den):
+        with np.errstate(invalid="ignore", divide="ignore"):
+            r = np.abs(a - b) / den
+        return np.where(np.isfinite(a) & np.isfinite(b), r, np.inf)
+
+    es = np.abs(q[:, j("jpot0")]) + np.abs(q[:, j("jkin0")]) + np.abs(q[:, j("jefp")])
+    e = {"ltf": rel(p[:, j("jltf")], q[:, j("jltf")], np.abs(q[:, j("jltf")]))}
+    for k, name in (("jetot", "elocal"), ("jpot0", "epot"), ("jkin0", "ekin"), ("jefp", "efp")):
+        e[name] = rel(p[:, j(k)], q[:, j(k)], np.maximum(np.abs(q[:, j(k)]), es))
+    e["e2"] = rel(p[:, j("je2")], q[:, j("je2")], es * es)
+    g, gr = np.asarray(out["g"]), np.asarray(ref["g"])
+    e["drift"] = np.max(np.abs(g - gr).reshape(len(g), -1), axis=1) / \
+        np.max(np.abs(gr).reshape(len(gr), -1), axis=1)
+    for t in range(ntypes):
+        if info["jkin"][t] == 0:
+            continue
+        for key, name in (("jkin", "kin"), ("jpot", "pot")):
+            k = info[key][t] - 1
+            if np.all(~np.isfinite(q[:, k])):
+                continue  # an empty type: 0/0 in the reference too
+            e[f"{name}{t + 1}"] = rel(p[:, k], q[:, k], np.maximum(np.abs(q[:, k]), es))
+        k = info["jun"][t] - 1
+        if np.any(np.isfinite(q[:, k])):
+            e[f"un{t + 1}"] = rel(p[:, k], q[:, k], np.abs(q[:, k]))
+    return e
+
+
+def per_type_g2(ref, counts):
+    """sum over each type of |g_i|^2 of a reference evaluation."""
+    g = np.asarray(ref["g"])
+    out, o = [], 0
+    for n in counts:
+        out.append(np.sum(g[:, o:o + n] ** 2, axis=(1, 2)))
+        o += n
+    return out
+
+
+def h_errors(out, ref, counts):
+    """Per-configuration errors of the per-type laplacian sums against
+    max(|h_t|, sum over the type of |g_i^ref|^2)."""
+    g2 = per_type_g2(ref, counts)
+    e = {}
+    for t, n in enumerate(counts):
+        if n == 0:
+            continue
+        a, b = np.asarray(out["h"])[:, t], np.asarray(ref["h"])[:, t]
+        e[f"h{t + 1}"] = np.abs(a - b) / np.maximum(np.abs(b), g2[t])
+    return e
+
+
 @pytest.fixture(scope="session")
 def oracle_mod():
     import pyoracle
