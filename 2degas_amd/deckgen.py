@@ -15,19 +15,14 @@ L = sqrt(pi N).
   into the scaled cubic of `spline` (reptation.f:5223-5245): t1 = f,
   t2 = f' drt, t3 = 3(f1-f0) - (t2_1 + 2 t2_0), t4 = -2(f1-f0) + (t2_1+t2_0),
   last row zero.
-* Ewald: the standard erfc split of e2/r in 2D.  Real space: table of
-  r v_sr(r) = e2 erfc(alpha r) read with iexp = 1; k space:
-  vk = (2 pi e2 / (A k)) erfc(k / 2 alpha) on the half-space list; the
-  self and background constants go into v0 and vk0 = 0.  This is a
-  textbook Ewald sum in place of the reference's optimized breakup fit
-  (setup.f:287-542); both give the same Coulomb energy to their truncation
-  errors.
+* Tables and the Ewald breakup come from the reference's own generators,
+  restated in 2degas_amd/egas.py (setup.f, reptation.f:5007-5245,
+  6517-6589, 9474-9512).
 """
 import math
 import os
 
 import numpy as np
-from scipy.special import erfc
 
 
 def klist(L, nshell):
@@ -85,83 +80,95 @@ def write_table(path, rows, drt, routine="deckgen", params=(), kspace=None):
             fh.write(f" {float(vk0)!r}\n")
 
 
-def _cut(r, R):
-    """Smooth cutoff (1 - r/R)^3 and its derivative, zero beyond R."""
-    x = np.clip(1.0 - r / R, 0.0, None)
-    return x ** 3, -3.0 * x ** 2 / R
+# Parameters of the tables the reference ships with its n26 deck (the
+# tails of n26/n26.u2, n26.u3, n26.b): u2_ob_cos (cusp, b, c, d, e and
+# three cosine amplitudes), u2_reatto (a..h), eta_yk (lambda, s, r_B, w).
+N26_U2 = (-1.0, -0.0667996019, -0.542445288, 0.323680102, 1.9738804, 0.123501158,
+          0.0286990313, -0.00104769406)
+N26_U2_FLAGS = (0, 1, 1, 1, 0, 1, 1, 1)
+N26_U3 = (0.0, 1.0, 8668.17812, 0.0863705028, -10.5950392, 0.0, 1.0, 1.0)
+N26_U3_FLAGS = (0, 0, 1, 1, 1, 0, 0, 0)
+N26_B = (0.57548746, -0.0453339702, 1.38305246, 3.33824371)
 
 
-def ewald_constants(N, L, e2, alpha, kv):
-    """k-space coefficients vk (half-space list, so sum_half vk |rho_k|^2 is
-    (1/2A) sum_{k != 0} v(k) |rho_k|^2) and v0 = self term
-    -N alpha e2/sqrt(pi) plus the neutralizing background
-    -N^2 e2 sqrt(pi) / (alpha A)."""
-    A = L * L
-    k = np.sqrt((kv ** 2).sum(1))
-    vk = 2.0 * math.pi * e2 / (A * k) * erfc(k / (2.0 * alpha))
-    self_e = -N * alpha * e2 / math.sqrt(math.pi)
-    bg = -N * N * e2 * math.sqrt(math.pi) / (alpha * A)
-    return vk, self_e + bg
+def write_v_table(path, rows, drt, vk):
+    """The pair-potential file traduci writes (setup.f:2250-2311): `nt drt`,
+    the rows, `fitpn`, `0`, `kspace`, the count and the k-space
+    coefficients, and vk0 = 0 (the constant goes to the deck's v0)."""
+    nt = rows.shape[0] - 1
+    with open(path, "w") as fh:
+        fh.write(f" {nt} {float(drt)!r}\n")
+        for r in rows:
+            fh.write(" " + " ".join(f"{float(v)!r}" for v in r) + "\n")
+        fh.write(" fitpn\n 0\n kspace\n")
+        fh.write(f" {len(vk)}\n")
+        for v in vk:
+            fh.write(f" {float(v)!r}\n")
+        fh.write(" 0.d0\n")
 
 
 def make_deck(dirpath, runid, nup, ndown, rs, nshell_ewald, nshell_k=None, ngrid=200,
-              ngrid_u3=100, u3=True, backflow=True, eta_amp=0.2, ntwists=0, seed=(3535, 3467, 3262, 1549),
-              param_seed=1, perturb=0.0, alpha=None):
-    """Write a complete deck; returns a dict of the physical parameters."""
+              ngrid_u3=100, u3=True, backflow="yk", eta_amp=0.2, ntwists=0,
+              seed=(3535, 3467, 3262, 1549), param_seed=1, perturb=0.0):
+    """Write a complete deck with the reference's own generators
+    (2degas_amd/egas.py); returns a dict of its physical parameters.
+
+    * box and units as egas's main program (setup.f:20-44): density 1/pi in
+      units of rs, L = (N pi)^(1/2), hbs2m = 1/rs^2, drt = L/(2 ngrid);
+    * u2: u2_ob_cos with the cusp -rs, as egas's main program builds it
+      (setup.f:108-112), and the other parameters of the shipped n26.u2;
+    * u3 (optional): u2_reatto with the parameters of n26.u3 on ngrid_u3
+      points of the same drt (one drt serves every table, reptation.f:4705);
+    * backflow: "yk" is eta_yk with the parameters of n26.b, "gauss" is
+      bozzo's eta_amp exp(-(r - 1)^2) (r0 = w = rs), "zero" an all-zero
+      table (the plain Slater-Jastrow function);
+    * v: ocpset's optimized Ewald breakup of e2/r over the first
+      nshell_ewald shells and v0 (egas.ocpset), written as traduci does.
+    perturb scales every free parameter by a factor uniform in
+    [1 - perturb, 1 + perturb] drawn from param_seed."""
+    from . import egas
     os.makedirs(dirpath, exist_ok=True)
     rng = np.random.default_rng(param_seed)
-    pert = lambda: 1.0 + perturb * rng.uniform(-1.0, 1.0)
+    pert = lambda v: v * (1.0 + perturb * rng.uniform(-1.0, 1.0)) if v != 0.0 else v
     N = nup + ndown
-    L = math.sqrt(math.pi * N)
+    L = (N / (1.0 / math.acos(-1.0))) ** (1.0 / 2.0)
     hbs2m = 1.0 / rs ** 2
-    e2 = 2.0 / rs
-    R = L / 2.0
-    drt = R / ngrid
-    r = np.arange(ngrid + 1) * drt
-    # two-body: RPA-like u(r) = A (1 - exp(-r/F)) / r, cut smoothly at L/2
-    Fc = math.sqrt(rs) * pert()
-    Aamp = math.sqrt(rs) * 0.8 * pert()
-    rr = np.where(r > 0, r, 1.0)
-    ex = np.exp(-rr / Fc)
-    base = np.where(r > 0, Aamp * (1.0 - ex) / rr, Aamp / Fc)
-    dbase = np.where(r > 0, Aamp * (ex / (Fc * rr) - (1.0 - ex) / rr ** 2), -Aamp / (2 * Fc * Fc))
-    c, dc = _cut(r, R)
-    u2f = base * c
-    u2d = dbase * c + base * dc
-    # three-body xi(r): smooth seeded Gaussians on ngrid_u3 points (the
-    # table ends at ngrid_u3*drt; beyond, the last zero row is used)
-    ru = r[: ngrid_u3 + 1]
-    Ru = ngrid_u3 * drt
-    cu, dcu = _cut(ru, Ru)
-    coef = rng.normal(0.0, 0.02, size=3)
-    wid = np.array([0.6, 1.2, 2.0]) * max(1.0, Ru / 6.0)
-    g = sum(cf * np.exp(-(ru / w) ** 2) for cf, w in zip(coef, wid))
-    dg = sum(cf * np.exp(-(ru / w) ** 2) * (-2.0 * ru / w ** 2) for cf, w in zip(coef, wid))
-    u3f = g * cu
-    u3d = dg * cu + g * dcu
-    # backflow eta(r) = a exp(-(r - 1)^2) in rs units (r0 = w = rs Bohr)
-    a = eta_amp * pert()
-    e = a * np.exp(-(r - 1.0) ** 2)
-    de = e * (-2.0 * (r - 1.0))
-    etaf = e * c
-    etad = de * c + e * dc
-    # Ewald
+    drt = 0.5 * L / ngrid
+    # the Ewald fit needs shells beyond nshell_ewald: the reference's k
+    # cutoffs (8 and 14, for 18 shells) grow with the last fitted shell
+    kv_e = klist(L, nshell_ewald + 1)
+    klast = float(np.sqrt((kv_e[-1] ** 2).sum()))
+    cutk, argek = max(8.0, 2.0 * klast), max(14.0, 3.4 * klast)
+    ew = egas.ocpset(N, rs, ngrid, nlamb=nshell_ewald, cutk=cutk, argek=argek)
     kv_e = klist(L, nshell_ewald)
-    kv = klist(L, nshell_k or nshell_ewald)
-    if alpha is None:
-        kmax = float(np.sqrt((kv_e ** 2).sum(1)).max())
-        alpha = kmax / (2.0 * 5.6)
-    vk, v0 = ewald_constants(N, L, e2, alpha, kv_e)
-    vtab = e2 * erfc(alpha * r)
-    dvtab = -e2 * 2.0 * alpha / math.sqrt(math.pi) * np.exp(-(alpha * r) ** 2)
-    write_table(os.path.join(dirpath, runid + ".u2"), spline_rows(u2f, u2d, drt), drt, "rpa", [Aamp, Fc])
+    if len(ew["vk"]) != len(kv_e):
+        raise ValueError("Ewald shells and k list disagree")
+    vrows = egas.spline(ew["vr"], ew["dvr"], drt, 1)
+    write_v_table(os.path.join(dirpath, runid + ".v"), vrows, drt, ew["vk"])
+    u2p = [N26_U2[0] * rs] + [pert(v) for v in N26_U2[1:]]
+    egas.write_tgen_table(os.path.join(dirpath, runid + ".u2"),
+                          egas.tgen("u2_ob_cos", u2p, ngrid, drt), drt, "u2_ob_cos", u2p,
+                          N26_U2_FLAGS)
+    u3p = None
     if u3:
-        write_table(os.path.join(dirpath, runid + ".u3"), spline_rows(u3f, u3d, drt), drt, "gauss3", list(coef))
-    write_table(os.path.join(dirpath, runid + ".b"), spline_rows(etaf if backflow else 0 * etaf,
-                                                              etad if backflow else 0 * etad, drt),
-                drt, "eta_gauss", [a])
-    write_table(os.path.join(dirpath, runid + ".v"), spline_rows(vtab, dvtab, drt), drt, "ewald_erfc",
-                [alpha], kspace=(vk, 0.0))
+        u3p = [pert(v) if fl else v for v, fl in zip(N26_U3, N26_U3_FLAGS)]
+        egas.write_tgen_table(os.path.join(dirpath, runid + ".u3"),
+                              egas.tgen("u2_reatto", u3p, ngrid_u3, drt), drt, "u2_reatto", u3p,
+                              N26_U3_FLAGS)
+    if backflow == "yk":
+        bp = [pert(v) for v in N26_B]
+        brows = egas.tgen("eta_yk", bp, ngrid, drt)
+        bname = "eta_yk"
+    elif backflow == "gauss":
+        bp = [pert(eta_amp), 1.0, 1.0]
+        brows = egas.tgen("bozzo", bp, ngrid, drt)
+        bname = "bozzo"
+    else:
+        bp = []
+        brows = np.zeros((ngrid + 1, 4))
+        bname = "zero"
+    egas.write_tgen_table(os.path.join(dirpath, runid + ".b"), brows, drt, bname, bp)
+    kv = klist(L, nshell_k or nshell_ewald)
     with open(os.path.join(dirpath, runid + ".k"), "w") as fh:
         for k in kv:
             fh.write(f" {float(k[0])!r} {float(k[1])!r}\n")
@@ -185,7 +192,7 @@ def make_deck(dirpath, runid, nup, ndown, rs, nshell_ewald, nshell_k=None, ngrid
         lines.append(f"ntwists {ntwists}")
     for p in pairs:
         lines.append(f"v2 {p[0]} {p[1]} {runid}.v 0. 1 1.")
-    lines.append(f"v0 {float(v0)!r}")
+    lines.append(f"v0 {float(ew['v0'] * N)!r}")
     lines.append("end")
     for ln in lines:
         if len(ln) > 80:
@@ -196,8 +203,8 @@ def make_deck(dirpath, runid, nup, ndown, rs, nshell_ewald, nshell_k=None, ngrid
         fh.write(runid + "\n")
     with open(os.path.join(dirpath, "seed"), "w") as fh:
         fh.write(" ".join(str(s) for s in seed) + "\n")
-    return dict(N=N, L=L, rs=rs, hbs2m=hbs2m, e2=e2, alpha=float(alpha), v0=float(v0), nk=len(kv),
-                nk_ewald=len(kv_e), drt=drt)
+    return dict(N=N, L=L, rs=rs, hbs2m=hbs2m, e2=2.0 / rs, v0=float(ew["v0"] * N), nk=len(kv),
+                nk_ewald=len(kv_e), drt=drt, ewald_rms=ew["rms"], u2=u2p, u3=u3p, b=bp)
 
 
 def random_configs(nconf, nptot, L, seed):

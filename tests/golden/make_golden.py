@@ -9,9 +9,16 @@ live under tests/golden/decks/:
   n26g     the same deck at the Gamma point (ntwists line dropped: no efp)
   n26p     plain Slater-Jastrow: the backflow table replaced by zeros, Gamma
   free26   free plane waves: zero backflow, no u2/u3, Gamma
-  g58      generated N=58 (29+29), rs=5, u2+u3+backflow, 60 k-shells
-  g122     generated N=122 (61+61), rs=10, u2+u3+backflow, 200 k-shells
+  c26      generated N=26 (13+13), rs=5: u2 alone, zero backflow,
+           60 k-shells (K=232), tables on 200 points (plain Slater-Jastrow)
+  g58      generated N=58 (29+29), rs=5, u2 + u3 (100 points) + the
+           Gaussian backflow 0.2 exp(-(r-1)^2), 60 k-shells
+  g58r1, g58r10   the same at rs=1 and rs=10
+  g122     generated N=122 (61+61), rs=10, u2+u3+backflow (eta_yk),
+           200 k-shells
   g61p     generated polarized N=61 (+ an empty second type), rs=10
+All generated decks come from the restated generators of the reference
+(2degas_amd/egas.py through deckgen.make_deck).
   n26r     n26 at the Gamma point without the rhok properties (reptation
            fixtures: 14 property slots per bead)
   n26a     n26 at the Gamma point with the drift cutoff on (adrift 0.5)
@@ -92,9 +99,12 @@ def derive_n26_variants():
 
 
 def make_generated():
-    deckgen.make_deck(os.path.join(DECKS, "g58"), "g58", 29, 29, 5.0, 60)
-    deckgen.make_deck(os.path.join(DECKS, "g122"), "g122", 61, 61, 10.0, 200)
-    deckgen.make_deck(os.path.join(DECKS, "g61p"), "g61p", 61, 0, 10.0, 200)
+    mk = deckgen.make_deck
+    mk(os.path.join(DECKS, "c26"), "c26", 13, 13, 5.0, 60, u3=False, backflow="zero")
+    for name, rs in (("g58", 5.0), ("g58r1", 1.0), ("g58r10", 10.0)):
+        mk(os.path.join(DECKS, name), name, 29, 29, rs, 60, backflow="gauss")
+    mk(os.path.join(DECKS, "g122"), "g122", 61, 61, 10.0, 200)
+    mk(os.path.join(DECKS, "g61p"), "g61p", 61, 0, 10.0, 200)
 
 
 def eval_fixture(name, nconf, seed, twist=(1, 1), extra=None, delta=0.0):
@@ -208,6 +218,9 @@ def main():
     eval_fixture("g58", 16, 17)
     eval_fixture("g122", 6, 18)
     eval_fixture("g61p", 6, 19)
+    eval_fixture("c26", 32, 23)
+    eval_fixture("g58r1", 16, 24)
+    eval_fixture("g58r10", 16, 25)
     # twist lists written by twistg for a few twists (pins the heap sort)
     gv = {}
     for ij in ((1, 1), (2, 3), (4, 4), (3, 1)):
@@ -224,7 +237,7 @@ def main():
     sites = np.array([(a, b) for a in g for b in g])
     rng = np.random.default_rng(21)
     x58 = sites[rng.permutation(64)[:58]] + rng.normal(0, 0.05 * L58 / 8, size=(58, 2))
-    replay_fixture("g58", 1000, 100, 2.0, x58)
+    replay_fixture("g58", 1000, 100, 0.5, x58)
 
 
 if __name__ == "__main__":

@@ -47,27 +47,40 @@ def test_spline_rows_reproduce_values_and_slopes():
     assert np.allclose((t[:-1, 1] + 2 * t[:-1, 2] + 3 * t[:-1, 3]) / drt, df[1:], atol=1e-12)
 
 
-def test_ewald_total_independent_of_alpha():
-    """The erfc split gives the same Coulomb energy for two splitting
-    parameters (real-space + k-space + constants)."""
-    dg = pkg("deckgen")
+def test_breakup_energy_matches_ewald_sum():
+    """A generated deck's Coulomb energy (optimized breakup: real-space
+    table with iexp = 1 plus the k-space coefficients plus v0) equals an
+    independent erfc Ewald sum of the same periodic system within the
+    breakup's fit accuracy (N = 14, rs = 2)."""
     from scipy.special import erfc
-    N = 14
-    L = math.sqrt(math.pi * N)
-    e2 = 0.4
+    import tempfile
+    dg = pkg("deckgen")
+    deck = pkg("deck")
+    with tempfile.TemporaryDirectory() as td:
+        info = dg.make_deck(td, "t14", 7, 7, 2.0, 30, ngrid=400, u3=False, backflow="zero")
+        d = deck.read_deck(td)
+    N, L, e2 = 14, info["L"], 2.0 / 2.0
     rng = np.random.default_rng(3)
     x = rng.uniform(-L / 2, L / 2, size=(N, 2))
+    # the deck's potential: sum over pairs of v_sr(r)/r from the spline
+    # rows (minimum image), sum_k vk |rho_k|^2 over the half space, and v0
+    t = [tb for tb in d.tables if tb.vk is not None][0]
+    dd = x[:, None, :] - x[None, :, :]
+    dd -= L * np.round(dd / L)
+    r = np.sqrt((dd ** 2).sum(-1))[np.triu_indices(N, 1)]
+    ind = np.minimum((r / t.drt).astype(int), t.coef.shape[0] - 1)
+    rm = (r - (r / t.drt).astype(int) * t.drt) / t.drt
+    c = t.coef[ind]
+    vr = c[:, 0] + rm * (c[:, 1] + rm * (c[:, 2] + rm * c[:, 3]))
+    rho = np.exp(1j * (x @ d.kvec.T)).sum(0)
+    ebreak = (vr / r).sum() + (t.vk * np.abs(rho) ** 2).sum() + d.v0
+    # erfc Ewald sum of the same system (alpha large enough for the
+    # minimum image in real space, many shells in k space)
+    alpha = 1.6
     kv = dg.klist(L, 400)
-    tot = []
-    for alpha in (1.3, 1.8):
-        vk, v0 = dg.ewald_constants(N, L, e2, alpha, kv)
-        d = x[:, None, :] - x[None, :, :]
-        d -= L * np.round(d / L)
-        r = np.sqrt((d ** 2).sum(-1))
-        iu = np.triu_indices(N, 1)
-        # minimum image suffices where erfc(alpha r) has decayed by L/2
-        er = (e2 * erfc(alpha * r[iu]) / r[iu]).sum()
-        rho = np.exp(1j * (x @ kv.T)).sum(0)
-        ek = (vk * np.abs(rho) ** 2).sum()
-        tot.append(er + ek + v0)
-    assert abs(tot[0] - tot[1]) < 1e-9 * abs(tot[0])
+    k = np.sqrt((kv ** 2).sum(1))
+    rho2 = np.abs(np.exp(1j * (x @ kv.T)).sum(0)) ** 2
+    A = L * L
+    eew = (e2 * erfc(alpha * r) / r).sum() + (2 * np.pi * e2 / (A * k) * erfc(k / (2 * alpha)) * rho2).sum() \
+        - N * alpha * e2 / np.sqrt(np.pi) - N * N * e2 * np.sqrt(np.pi) / (alpha * A)
+    assert abs(ebreak - eew) < 2e-5 * N * e2, (ebreak, eew)

@@ -16,7 +16,8 @@ RTOL = 1e-10
 EVAL_FIXTURES = ["eval_n26_t11.npz", "eval_n26_t23.npz", "eval_n26_t44.npz",
                  "eval_n26g_t11.npz", "eval_n26p_t11.npz", "eval_free26_t11.npz",
                  "eval_g58_t11.npz", "eval_g122_t11.npz", "eval_g61p_t11.npz",
-                 "eval_n26a_t11.npz"]
+                 "eval_n26a_t11.npz", "eval_c26_t11.npz", "eval_g58r1_t11.npz",
+                 "eval_g58r10_t11.npz"]
 
 _ctx_cache = {}
 

@@ -13,7 +13,8 @@ from conftest import compare_eval, load_deck, load_golden, pkg
 EVAL_FIXTURES = ["eval_n26_t11.npz", "eval_n26_t23.npz", "eval_n26_t44.npz",
                  "eval_n26g_t11.npz", "eval_n26p_t11.npz", "eval_free26_t11.npz",
                  "eval_g58_t11.npz", "eval_g122_t11.npz", "eval_g61p_t11.npz",
-                 "eval_n26a_t11.npz"]
+                 "eval_n26a_t11.npz", "eval_c26_t11.npz", "eval_g58r1_t11.npz",
+                 "eval_g58r10_t11.npz"]
 
 
 def _deck_of(fn):
