@@ -128,6 +128,31 @@ class Oracle:
         return dict(p=p, age=age, ltf_new=ltf, e_new=e, x_final=xf, seed_final=sf)
 
 
+def _pool_eval(args):
+    deckdir, x, extended, twist, delta = args
+    d = _pkg.deck.read_deck(deckdir)
+    o = Oracle(d, twist=twist, extended=extended)
+    if delta is not None:
+        o.set_delta(delta)
+    return o.eval(x)
+
+
+def eval_parallel(deckdir, x, extended=False, nproc=None, twist=(1, 1), delta=None):
+    """Oracle evaluation of many configurations in worker processes started
+    fresh (spawn: no GPU state is inherited), nproc of them (default: the
+    CPUs of this process, at most 16)."""
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing as mp
+    if nproc is None:
+        nproc = max(1, min(16, len(os.sched_getaffinity(0))))
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    chunks = np.array_split(x, min(len(x), nproc * 4))
+    with ProcessPoolExecutor(nproc, mp_context=mp.get_context("spawn")) as ex:
+        parts = list(ex.map(_pool_eval, [(deckdir, c, extended, tuple(twist), delta)
+                                         for c in chunks]))
+    return {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+
+
 def rank_seed_limbs(seed, rank):
     """RNG limbs of rank `rank` after input's seeding
     (reptation.f:37-39: seed(4)=2*(seed(4)+mytid)+1, setrn)."""

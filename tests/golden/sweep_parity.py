@@ -64,6 +64,10 @@ def _vmc_chain(args):
     raw, _ = po.run_refeval(os.path.join(DECKS, name), os.path.join(WORK, f"{name}_v{k}"), 2,
                             x0[None], delta=delta, nsteps=nmoves, seed=seed)
     r = po.parse_replay(raw, d.nptot, nmoves)
+    acc = float(np.mean(r["age"][200:] == 0))
+    if acc < 0.2:
+        raise RuntimeError(f"VMC chain {k} of {name} accepts {acc:.3f} of its moves at delta "
+                           f"{delta}: lower the time step")
     return r["x_old"][200::10]
 
 
@@ -99,7 +103,9 @@ def main(name, nunif, nvmc, delta):
     ref_full = {"props": ref["scalars"], "g": ref["g"],
                 "h": np.pad(ref["h"], ((0, 0), (0, 2 - ref["h"].shape[1])))}
     counts = [d.np[t] for t in range(d.ntypes)]
+    nuniq = len(np.unique(x[nunif:].reshape(len(x) - nunif, -1), axis=0)) if len(x) > nunif else 0
     res = {"deck": name, "nptot": d.nptot, "n_uniform": nunif, "n_vmc": int(len(x) - nunif),
+           "n_vmc_distinct": int(nuniq),
            "vmc_delta": delta, "compiler": open(os.path.join(po.REF_DIR, "COMPILER")).read().strip()}
     for tag, a, b in (("ref_vs_exact", ref_full, exa), ("dbl_vs_exact", dbl, exa),
                       ("dbl_vs_ref", dbl, ref_full)):

@@ -14,8 +14,10 @@ from conftest import compare_eval, pkg
 pytestmark = pytest.mark.gpu
 
 RTOL = 1e-10
+# 16, 17, 32, 33, 48 and 49 per spin are the tile edges of the matrix-core
+# determinant: k_gjm_part's tile skipping below 49 and the switch to k_gjm
 SIZES = [(33, 33), (37, 29), (40, 20), (48, 47), (57, 0), (64, 64), (64, 1), (20, 12),
-         (29, 29), (13, 13), (5, 3), (37, 0)]
+         (29, 29), (13, 13), (5, 3), (37, 0), (16, 16), (17, 15), (32, 31), (49, 48)]
 LU_SIZES = [(20, 12), (29, 29), (37, 29)]
 
 
