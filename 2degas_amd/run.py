@@ -1,9 +1,10 @@
-"""Batched VMC runs with the reference's inputs and outputs.
+"""Batched VMC and RQMC runs with the reference's inputs and outputs.
 
-`run(dirpath, nw)` does for the `vmc` command of `<runid>.in` what the
-reference's driver does (sonaseppia, reptation.f:712-754; vmc,
-reptation.f:973-1010), for a batch of nw walkers on the GPU, walker k playing
-MPI rank k of the reference:
+`run(dirpath, nw)` does for the `vmc` and `rmc` commands of `<runid>.in`
+what the reference's driver does (sonaseppia, reptation.f:712-807; vmc,
+reptation.f:973-1010; rmc, reptation.f:1286-1313), for a batch of nw
+walkers (or reptiles) on the GPU, walker k playing MPI rank k of the
+reference:
 
 * inputs: `runid`; `seed` (4 limbs; rank k draws from the limbs
   (s1, s2, s3, 2 (s4 + k) + 1), the second generator keeps (s1..s4),
@@ -23,7 +24,15 @@ MPI rank k of the reference:
   `<x_file>.<k>` (write_conf, reptation.f:1008-1011, 2503-2535);
 * a deck with `ntwists n` runs the blocks once per twist (ith, jth) in
   1..n x 1..n, as tabc does (reptation.f:9559-9638), each twist reading
-  the configurations the previous one wrote.
+  the configurations the previous one wrote;
+* an `rmc` record reads each rank's initial path of ntau configurations
+  from `<x_file>.<k>` (read_path, reptation.f:1766-1802), prints
+  `rmc: nprops =`, resets the reptile age at block 1 (averages(1),
+  reptation.f:4290), writes `===>> rmc block` lines, `.res` with the path
+  from jfirst in `<x_file>.res.<k>` after every block (restart(1),
+  reptation.f:4481-4490) and the final path to `<x_file>.<k>` (write_path,
+  reptation.f:1741-1764); after a `restart` record the trial energy is the
+  cumulative mean energy of the restart file (restart(0), reptation.f:4525).
 
 Block averages sum p_old over the walkers and steps of the block, the
 MPI_REDUCE of the reference (reptation.f:4308-4311), so that blk_norm is
@@ -162,13 +171,36 @@ class Run:
         return os.path.join(self.dir, name)
 
     # -- files
-    def read_configs(self):
+    def read_configs(self, nconf=1):
+        """nconf configurations per rank from `<x_file><res_string><k>`
+        (read_conf, reptation.f:2452-2495): [nw][N][2], or [nw][nconf][N][2]
+        when nconf > 1."""
         d = self.deck
         x = []
         for k in range(self.nw):
             paths = [self._path(f + self.res_string + str(k)) for f in d.x_file]
-            x.append(deckmod.read_configs(paths, d.np, 1)[0])
+            c = deckmod.read_configs(paths, d.np, nconf)
+            if len(c) < nconf:
+                raise ValueError(f"rank {k}: {len(c)} configurations in {paths}, {nconf} needed")
+            x.append(c[0] if nconf == 1 else c[:nconf])
         return np.stack(x)
+
+    def write_paths(self, x, ends, mid):
+        """Each rank's path from jfirst, ntau beads, in `<x_file><mid><k>`
+        (write_path, reptation.f:1741-1764; restart(1) for rmc,
+        reptation.f:4481-4490): per type file, the beads in path order."""
+        d = self.deck
+        nr, nbuf = x.shape[0], x.shape[1]
+        off = 0
+        for it, f in enumerate(d.x_file):
+            n = d.np[it]
+            for k in range(nr):
+                with open(self._path(f"{f}{mid}{k}"), "w") as fh:
+                    for i in range(self.ntau):
+                        b = (int(ends[k, 0]) + i) % nbuf
+                        for p in x[k, b, off:off + n]:
+                            fh.write(f" {_ld(p[0])} {_ld(p[1])}\n")
+            off += n
 
     def write_configs(self, x, mid=".res."):
         """`<x_file><mid><k>`: `.res.` after every block (restart(1)), `.`
@@ -195,7 +227,7 @@ class Run:
             if self.deck.ntheta != 0:
                 f.write(f" {self.twist[0]} {self.twist[1]}\n")
 
-    def read_restart(self, nblk, n_props):
+    def read_restart(self, nblk, n_props, who="vmc"):
         with open(self._path(self.runid + ".res")) as f:
             tok = f.read().split()
         iblk = int(tok[0])
@@ -296,6 +328,53 @@ class Run:
             finally:
                 ctx.close()
 
+    # -- the rmc command
+    def rmc(self, nblk, nstp, delta, ntau, ntauskip, mdelta, etrial, mage, mage_r):
+        import torch
+        d = self.deck
+        self.ntau = ntau
+        # rmc runs on the g-vectors input set up, without the twist loop
+        # (sonaseppia calls rmc directly, reptation.f:795-807)
+        gv = hip.twist_gvec(d.kvec, d.el, 0, 0, 0)
+        ctx = hip.Context(d, gvec=gv)
+        try:
+            info = ctx.info
+            names = scalar_names(d, info)
+            if self.res_string != ".":
+                self.read_restart(nblk, info["n_props"], who="rmc")
+                # restart(0) takes the trial energy from the cumulative
+                # average (reptation.f:4525)
+                etrial = self.stats.av[info["jetot"] - 1] / self.stats.norm
+            prm = hip.rmc_params(delta, etrial, mage, mage_r, ntauskip, mdelta, d.driver)
+            beads = self.read_configs(ntau)
+            states = [abi.limbs_to_state(l) for l in self.limbs]
+            rep = hip.Reptiles(ctx, self.nw, ntau, mdelta, states)
+            rep.start(beads, prm)
+            self.out.write(f" rmc: nprops = {info['n_props']:12d}\n")
+            for iblk in range(self.iblk0, nblk + 1):
+                if iblk == 1 or self.stats is None:
+                    self.stats = _Stats(info["n_props"])
+                    # averages(1) at block 1 resets the reptile age
+                    # (reptation.f:4286-4290)
+                    rep.rage.zero_()
+                rep.blk.zero_()
+                rep.sweep(nstp, prm)
+                torch.cuda.synchronize()
+                st = rep.status()
+                if np.any(st & 2):
+                    raise RuntimeError("singular orbital matrix (the reference stops here, reptation.f:10318)")
+                blk = rep.blk.sum(dim=0).cpu().numpy()
+                blk_norm = float(self.nw * nstp)
+                self.stats.add(blk, blk_norm)
+                self.limbs = [abi.state_to_limbs(int(v)) for v in rep.rng.cpu().numpy()]
+                self.block_output(iblk, blk, blk_norm, names, info, who="rmc")
+                self.write_restart(iblk, who="rmc")
+                self.write_paths(rep.x.cpu().numpy(), rep.ends.cpu().numpy(), ".res.")
+            self.write_paths(rep.x.cpu().numpy(), rep.ends.cpu().numpy(), ".")
+            rep = None
+        finally:
+            ctx.close()
+
     def execute(self):
         """Run the records of `<runid>.in`."""
         recs = read_script(self.dir, self.runid)
@@ -318,7 +397,16 @@ class Run:
                     self.read_restart(nblk, self._n_props())
                 self.vmc(nblk, nstp, delta)
                 continue
-            raise NotImplementedError(f"{w[0]}: only vmc, restart and end records run on this path")
+            if w[0] == "rmc":
+                # rmc nblk nstp delta ntau ntauskip mdelta etrial mage mage_r
+                # (reptation.f:795-805)
+                nblk, nstp = int(w[1]), int(w[2])
+                delta = deckmod._ffloat(w[3])
+                ntau, ntauskip, mdelta = int(w[4]), int(w[5]), int(w[6])
+                etrial, mage, mage_r = (deckmod._ffloat(v) for v in w[7:10])
+                self.rmc(nblk, nstp, delta, ntau, ntauskip, mdelta, etrial, mage, mage_r)
+                continue
+            raise NotImplementedError(f"{w[0]}: only vmc, rmc, restart and end records run on this path")
         self.write_seed()
 
     def _n_props(self):

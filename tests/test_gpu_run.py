@@ -199,3 +199,117 @@ def test_twist_loop_matches_reference(tmp_path):
     # accept/reject sequence is the same, as the final generator limbs show
     _close(np.array(vo), np.array(vr), 1e-6, "per-twist block averages")
     assert tr[-10:] == to[-10:]  # the 8 limbs and the twist (4, 4)
+
+
+# -- rmc records ------------------------------------------------------------
+
+RMC_REC = "rmc {nblk} 12 0.05 16 3 4 -0.45 1000000 1000000\n"
+
+
+def _rmc_deck(work, seed=None, paths=None):
+    """n26g with an initial path of 16 beads per rank k in <x_file>.k."""
+    shutil.copytree(os.path.join(DECKS, "n26g"), work, copy_function=shutil.copyfile)
+    for t in TYPES:
+        os.remove(os.path.join(work, f"n26.{t}.x.res.0"))
+    if seed is not None:
+        with open(os.path.join(work, "seed"), "w") as f:
+            f.write(" " + " ".join(str(v) for v in seed) + "\n")
+    for k, beads in paths.items():
+        off = 0
+        for t in TYPES:
+            with open(os.path.join(work, f"n26.{t}.x.{k}"), "w") as f:
+                for b in beads:
+                    for p in b[off:off + 13]:
+                        f.write(f" {float(p[0])!r} {float(p[1])!r}\n")
+            off += 13
+
+
+def _blocks_rmc(text):
+    return _blocks(text.replace("===>> rmc block", "===>> vmc block"))
+
+
+def _paths(nr, x0, seed):
+    rng = np.random.default_rng(seed)
+    out = {}
+    for k in range(nr):
+        steps = rng.normal(0.0, 0.02, size=(15,) + x0.shape)
+        out[k] = np.concatenate([x0[None], x0[None] + np.cumsum(steps, axis=0)])
+    return out
+
+
+@NEEDS_REF
+def test_rmc_one_reptile_reproduces_reference_driver_files(tmp_path):
+    """An rmc record (reptation.f:795-807, rmc 1286-1313) with one reptile:
+    stdout block lines, `.res` sums and limbs, the path from jfirst in
+    `<x_file>.res.0` after each block and in `<x_file>.0` at the end
+    (write_path), and `seed`; then a restart record continues both runs for
+    one more block (the trial energy taken from the restart file)."""
+    print("\n[gpu] batched rmc driver vs reference driver, 1 reptile", flush=True)
+    d = pkg("deck").read_deck(os.path.join(DECKS, "n26g"))
+    x0 = pkg("deck").read_configs([os.path.join(DECKS, "n26g", f"n26.{t}.x.res.0") for t in TYPES],
+                                  [13, 13])[0]
+    paths = _paths(1, x0, 77)
+    ref, ours = str(tmp_path / "ref"), str(tmp_path / "ours")
+    for w in (ref, ours):
+        _rmc_deck(w, paths=paths)
+        _script(w, RMC_REC.format(nblk=2) + "end\n")
+    out_r, out_o = _ref(ref), _ours(ours, 1)
+    assert " rmc: nprops =" in out_o
+    br, bo = _blocks_rmc(out_r), _blocks_rmc(out_o)
+    assert sorted(br) == sorted(bo) == [1, 2]
+    for b in br:
+        assert [n for _, n in br[b]] == [n for _, n in bo[b]]
+        _close([v for v, _ in bo[b]], [v for v, _ in br[b]], 1e-8, f"rmc block {b}")
+    tr, to = _res(ref), _res(ours)
+    assert tr[:2] == to[:2] == ["2", "rmc"] and len(tr) == len(to)
+    _close([float(v) for v in to[2:-8]], [float(v) for v in tr[2:-8]], 1e-8, "restart sums")
+    assert tr[-8:] == to[-8:]
+    for t in TYPES:
+        for mid in (".res.", "."):
+            a = np.loadtxt(os.path.join(ours, f"n26.{t}.x{mid}0"))
+            b = np.loadtxt(os.path.join(ref, f"n26.{t}.x{mid}0"))
+            assert a.shape == b.shape == (16 * 13, 2)
+            _close(a, b, 1e-10, f"{t} path {mid}")
+    with open(os.path.join(ref, "seed")) as f, open(os.path.join(ours, "seed")) as g:
+        assert f.read().split() == g.read().split()
+    for w in (ref, ours):
+        _script(w, "restart 1\n" + RMC_REC.format(nblk=3) + "end\n")
+    out_r, out_o = _ref(ref), _ours(ours, 1)
+    br, bo = _blocks_rmc(out_r), _blocks_rmc(out_o)
+    assert sorted(br) == sorted(bo) == [3]
+    _close([v for v, _ in bo[3]], [v for v, _ in br[3]], 1e-8, "rmc block 3 after restart")
+    assert _res(ref)[-8:] == _res(ours)[-8:]
+
+
+@NEEDS_REF
+def test_rmc_reptiles_are_ranks(tmp_path):
+    """Three reptiles = three ranks: reptile k of the batch is a single-rank
+    reference rmc run whose seed's last limb is offset by k, started from
+    path file k; block means are the mean of the three runs', and each
+    reptile's generator limbs and final path are that run's."""
+    print("\n[gpu] batched rmc driver, 3 reptiles vs 3 single-rank reference runs", flush=True)
+    x0 = pkg("deck").read_configs([os.path.join(DECKS, "n26g", f"n26.{t}.x.res.0") for t in TYPES],
+                                  [13, 13])[0]
+    paths = _paths(3, x0, 78)
+    seed = [3535, 3467, 3262, 1549]
+    ours = str(tmp_path / "ours")
+    _rmc_deck(ours, seed, paths)
+    _script(ours, RMC_REC.format(nblk=2) + "end\n")
+    bo = _blocks_rmc(_ours(ours, 3))
+    limbs = _res(ours)[-24:]
+    means = {b: [] for b in (1, 2)}
+    for k in range(3):
+        ref = str(tmp_path / f"ref{k}")
+        _rmc_deck(ref, seed[:3] + [seed[3] + k], {0: paths[k]})
+        _script(ref, RMC_REC.format(nblk=2) + "end\n")
+        br = _blocks_rmc(_ref(ref))
+        for b in (1, 2):
+            means[b].append([v[0] for v, _ in br[b]])
+        assert _res(ref)[-8:-4] == limbs[8 * k:8 * k + 4]
+        for t in TYPES:
+            a = np.loadtxt(os.path.join(ours, f"n26.{t}.x.{k}"))
+            b = np.loadtxt(os.path.join(ref, f"n26.{t}.x.0"))
+            _close(a, b, 1e-10, f"reptile {k} {t} path")
+    for b in (1, 2):
+        _close([v[0] for v, _ in bo[b]], np.mean(np.array(means[b]), axis=0), 1e-8,
+               f"rmc block {b} means")
