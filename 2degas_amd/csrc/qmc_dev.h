@@ -2,7 +2,8 @@
 //
 // One DevSys value (kernel argument) carries the system description and the
 // device pointers of the deck tables; the per-walker workspace layout is
-// described by WsLayout.  All arithmetic here is FP64.
+// described by WsLayout.  Arithmetic is FP64, except the FP32 table
+// evaluations the mixed-precision pair kernel selects (spline3m, spline0<true>).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -32,8 +33,10 @@ struct DevSys {
   int ntypes, nptot, npnorm;
   int np[2], ipf[2];
   double hbs2m[2], el[2], eli[2], drt, drti, drti2;
-  // spline tables: rows of 4 coefficients, table t row i at tab[t*tab_ld + i]
+  // spline tables: rows of 4 coefficients, table t row i at tab[t*tab_ld + i];
+  // tab32 holds the same rows rounded to FP32 (the mixed-precision path)
   const double4* tab;
+  const float4* tab32;
   int tab_ld;
   int ngrid[QMC_MAXTAB];
   int iexp[QMC_MAXTAB];
@@ -197,13 +200,39 @@ __device__ __forceinline__ void spline3(const DevSys& s, TabRef t, int ind, doub
   ddf = (c.z + rm * (3.0 * c.w)) * s.drti2;
 }
 
+// spline3 on the FP32 copy of the tables when MIX: the cubic and its
+// derivatives in FP32 from the FP64 grid remainder, returned widened to FP64.
+template <bool MIX>
+__device__ __forceinline__ void spline3m(const DevSys& s, TabRef t, int ind, double rem, double& f,
+                                         double& df, double& ddf) {
+  if (!MIX) {
+    spline3(s, t, ind, rem, f, df, ddf);
+    return;
+  }
+  int i = ind < t.ng ? (ind > 0 ? ind : 0) : t.ng;
+  const float4 c = s.tab32[t.row0 + i];
+  const float rm = (float)(rem * s.drti);
+  const float di = (float)s.drti;
+  f = (double)(c.x + rm * (c.y + rm * (c.z + rm * c.w)));
+  df = (double)((c.y + rm * (2.0f * c.z + rm * (3.0f * c.w))) * di);
+  ddf = (double)((c.z + rm * (3.0f * c.w)) * (float)s.drti2);
+}
+
 // Value only, times (1/r)^iexp (getf_0, reptation.f:4103-4112).
+template <bool MIX = false>
 __device__ __forceinline__ double spline0(const DevSys& s, TabRef t, int e, int ind, double rem,
                                          double byr) {
   int i = ind < t.ng ? (ind > 0 ? ind : 0) : t.ng;
-  double4 c = s.tab[t.row0 + i];
-  double rm = rem * s.drti;
-  double f = c.x + rm * (c.y + rm * (c.z + rm * c.w));
+  double f;
+  if (MIX) {
+    const float4 c = s.tab32[t.row0 + i];
+    const float rm = (float)(rem * s.drti);
+    f = (double)(c.x + rm * (c.y + rm * (c.z + rm * c.w)));
+  } else {
+    const double4 c = s.tab[t.row0 + i];
+    const double rm = rem * s.drti;
+    f = c.x + rm * (c.y + rm * (c.z + rm * c.w));
+  }
   double p = 1.0;
   int n = e < 0 ? -e : e;
   for (int k = 0; k < n; k++) p *= byr;

@@ -33,6 +33,7 @@ struct qmc_ctx {
   DevSys s{};
   qmc_info info{};
   double4* d_tab = nullptr;
+  float4* d_tab32 = nullptr;  // the tables rounded to FP32 (mixed precision)
   double* d_kvec = nullptr;
   double* d_vk = nullptr;
   double* d_gvec = nullptr;
@@ -53,6 +54,8 @@ struct qmc_ctx {
   double delta = 0.0;  // current time step (COMMON delta) for the drift cutoff
   int kc = 16;
   size_t lds_pairs = 0, lds_det = 0, lds_cross = 0;
+  size_t lds_pairs_mix = 0;
+  int kchunk_mix = 1;
   int tpp_det = 0;
   bool use_gj = false;   // matrix-core Gauss-Jordan determinant path (else k_det)
   bool gjm_part = false; // a spin below 49 particles: k_gjm_part, which skips the tiles beyond m
@@ -74,9 +77,11 @@ struct qmc_ctx {
 
 // Queue one batched evaluation (k_pairs, k_gjm or k_gjm_part and k_fgrad, or
 // k_det, then k_cross, k_final) on st.
-// delta is the time step the drift cutoff of kinetic uses (reptation.f:2755).
+// delta is the time step the drift cutoff of kinetic uses (reptation.f:2755);
+// precision QMC_PREC_MIXED selects the mixed-precision pair kernel.
 int launch_eval(qmc_ctx* c, int nw, const double* d_x, double* d_props, double* d_g, double* d_h,
-                double* d_g2, int32_t* d_status, hipStream_t st, double delta);
+                double* d_g2, int32_t* d_status, hipStream_t st, double delta,
+                int precision = QMC_PREC_FP64);
 void free_vmc(qmc_ctx* c);
 int ensure_vmc_scratch(qmc_ctx* c, int nw);
 void free_rmc(qmc_ctx* c);

@@ -18,6 +18,8 @@
 //             as a symmetric contraction of S = V V^T against M = Re(F o F^T).
 //   k_final   kinetic and potential energies and the property slots
 //             (kinetic, reptation.f:2729-2775; potential tail, 4088-4096).
+#include <type_traits>
+
 #include "qmc_dev.h"
 
 // k-vectors per thread and pass in k_pairs
@@ -58,10 +60,17 @@ __device__ __forceinline__ void pp_add(double* part, int N, int p, const double 
     if (q < nq) part[q * N + p] += v[q];
 }
 
-__global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs(DevSys s, WsLayout L,
-                                                     const double* __restrict__ x,
-                                                     double* __restrict__ ws,
-                                                     double* __restrict__ props) {
+// MIX (the mixed-precision evaluation): the u2, u3 and pair-potential
+// tables are read from their FP32 copy and evaluated in FP32 (spline3m), and
+// the rho_k phases are stored in LDS as FP32 pairs and multiplied and summed
+// over each chunk of particles in FP32, the chunk sums added in FP64; the
+// pair geometry, the backflow coordinates, every accumulator and the k-space
+// sum stay FP64.
+template <bool MIX>
+__device__ __forceinline__ void k_pairs_body(const DevSys& s, const WsLayout& L,
+                                             const double* __restrict__ x,
+                                             double* __restrict__ ws,
+                                             double* __restrict__ props) {
   extern __shared__ __attribute__((aligned(16))) double sm[];
   const int w = blockIdx.x;
   const int N = s.nptot;
@@ -109,7 +118,7 @@ __global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs(DevSys s, WsLayout L,
         double hh = 0.0;
         double f, df, ddf;
         if (tu2.row0 >= 0) {
-          spline3(s, tu2, p.ind, p.rem, f, df, ddf);
+          spline3m<MIX>(s, tu2, p.ind, p.rem, f, df, ddf);
           acc_ltf += f;
           const double dtr = df * p.byr;
           ja[0] -= dtr * p.ux;
@@ -117,7 +126,7 @@ __global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs(DevSys s, WsLayout L,
           hh += ddf + dtr;
         }
         if (tu3.row0 >= 0) {
-          spline3(s, tu3, p.ind, p.rem, f, df, ddf);
+          spline3m<MIX>(s, tu3, p.ind, p.rem, f, df, ddf);
           ja[2] += f * p.ux;
           ja[3] += f * p.uy;
           // (xi r)^2 pair term of the three-body factor (reptation.f:2822-2832)
@@ -141,7 +150,7 @@ __global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs(DevSys s, WsLayout L,
         if (ti == 0) acc_h0 += hh; else acc_h1 += hh;
         if (tj == 0) acc_h0 += hh; else acc_h1 += hh;
         if (tv.row0 >= 0) {
-          const double vv = 0.5 * spline0(s, tv, T->vexp, p.ind, p.rem, p.byr) * T->v2value;
+          const double vv = 0.5 * spline0<MIX>(s, tv, T->vexp, p.ind, p.rem, p.byr) * T->v2value;
           if (ti == 0) acc_v0 += vv; else acc_v1 += vv;
           if (tj == 0) acc_v0 += vv; else acc_v1 += vv;
         }
@@ -196,7 +205,7 @@ __global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs(DevSys s, WsLayout L,
         const double2 xi2 = ld2(xs, i), xj2 = ld2(xs, j);
         PairGeo p = pair_geo(s, xi2.x, xi2.y, xj2.x, xj2.y);
         double t, dt, ddt;
-        spline3(s, tu3, p.ind, p.rem, t, dt, ddt);
+        spline3m<MIX>(s, tu3, p.ind, p.rem, t, dt, ddt);
         const double c = dt * p.byr;
         const double m11 = t + c * p.ux * p.ux, m12 = c * p.ux * p.uy, m22 = t + c * p.uy * p.uy;
         const double2 Gi = ld2(Gs, i), Gj = ld2(Gs, j);
@@ -256,7 +265,8 @@ __global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs(DevSys s, WsLayout L,
       if (s.nkxv > 0) {
         // e^{i k.x} = e^{i kx x} e^{i ky y}: per particle chunk, the distinct
         // x and y phases go to LDS, then each thread multiplies its k's pairs
-        double2* ph = (double2*)(sm + 4 * N + 8);
+        using ph_t = typename std::conditional<MIX, float2, double2>::type;
+        ph_t* ph = (ph_t*)(sm + 4 * N + 8);
         const int nv = s.nkxv + s.nkyv;
         for (int c0 = 0; c0 < N; c0 += s.kchunk) {
           const int cn = min(s.kchunk, N - c0);
@@ -274,15 +284,15 @@ __global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs(DevSys s, WsLayout L,
               double sa, ca, sd, cd;
               qsincos_sreg((isx ? s.kxv[v0] : s.kyv[v0]) * u, &sa, &ca);
               qsincos_sreg((isx ? s.dkx : s.dky) * u, &sd, &cd);
-              double2* dst = ph + pc * nv + (isx ? 0 : s.nkxv) + v0;
-              dst[0] = make_double2(ca, sa);
+              ph_t* dst = ph + pc * nv + (isx ? 0 : s.nkxv) + v0;
+              dst[0] = ph_t{(decltype(ph_t::x))ca, (decltype(ph_t::x))sa};
 #pragma unroll
               for (int r = 1; r < KRUN; r++) {
                 if (r < nrun) {
                   const double c2 = fma(ca, cd, -sa * sd);
                   sa = fma(ca, sd, sa * cd);
                   ca = c2;
-                  dst[r] = make_double2(ca, sa);
+                  dst[r] = ph_t{(decltype(ph_t::x))ca, (decltype(ph_t::x))sa};
                 }
               }
             }
@@ -293,7 +303,7 @@ __global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs(DevSys s, WsLayout L,
               const double arg = v < s.nkxv ? s.kxv[v] * xs[2 * i] : s.kyv[v - s.nkxv] * xs[2 * i + 1];
               double sv, cv;
               qsincos_sreg(arg, &sv, &cv);
-              ph[e] = make_double2(cv, sv);
+              ph[e] = ph_t{(decltype(ph_t::x))cv, (decltype(ph_t::x))sv};
             }
           }
           __syncthreads();
@@ -301,16 +311,40 @@ __global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs(DevSys s, WsLayout L,
           // type a loop of its own
           const int split = s.ntypes > 1 ? min(max(s.ipf[1] - c0, 0), cn) : cn;
           auto accum = [&](int pa, int pb, double (&rc)[KMAXPT], double (&rs)[KMAXPT]) {
-            for (int pc = pa; pc < pb; pc++) {
-              const double2* row = ph + pc * nv;
+            if constexpr (!MIX) {
+              for (int pc = pa; pc < pb; pc++) {
+                const ph_t* row = ph + pc * nv;
 #pragma unroll
-              for (int h = 0; h < KMAXPT; h += 2) {
-                // one ky phase for the item's two kx phases
-                const double2 b = row[oy[h]], a0 = row[ox[h]], a1 = row[ox[h + 1]];
-                rc[h] = fma(a0.x, b.x, fma(-a0.y, b.y, rc[h]));
-                rs[h] = fma(a0.x, b.y, fma(a0.y, b.x, rs[h]));
-                rc[h + 1] = fma(a1.x, b.x, fma(-a1.y, b.y, rc[h + 1]));
-                rs[h + 1] = fma(a1.x, b.y, fma(a1.y, b.x, rs[h + 1]));
+                for (int h = 0; h < KMAXPT; h += 2) {
+                  // one ky phase for the item's two kx phases
+                  const ph_t b = row[oy[h]], a0 = row[ox[h]], a1 = row[ox[h + 1]];
+                  rc[h] = fma(a0.x, b.x, fma(-a0.y, b.y, rc[h]));
+                  rs[h] = fma(a0.x, b.y, fma(a0.y, b.x, rs[h]));
+                  rc[h + 1] = fma(a1.x, b.x, fma(-a1.y, b.y, rc[h + 1]));
+                  rs[h + 1] = fma(a1.x, b.y, fma(a1.y, b.x, rs[h + 1]));
+                }
+              }
+            } else {
+              // the chunk's products summed in FP32, the chunk sum added to
+              // the FP64 accumulators
+              float qc[KMAXPT], qs[KMAXPT];
+#pragma unroll
+              for (int h = 0; h < KMAXPT; h++) qc[h] = qs[h] = 0.0f;
+              for (int pc = pa; pc < pb; pc++) {
+                const ph_t* row = ph + pc * nv;
+#pragma unroll
+                for (int h = 0; h < KMAXPT; h += 2) {
+                  const ph_t b = row[oy[h]], a0 = row[ox[h]], a1 = row[ox[h + 1]];
+                  qc[h] = fmaf(a0.x, b.x, fmaf(-a0.y, b.y, qc[h]));
+                  qs[h] = fmaf(a0.x, b.y, fmaf(a0.y, b.x, qs[h]));
+                  qc[h + 1] = fmaf(a1.x, b.x, fmaf(-a1.y, b.y, qc[h + 1]));
+                  qs[h + 1] = fmaf(a1.x, b.y, fmaf(a1.y, b.x, qs[h + 1]));
+                }
+              }
+#pragma unroll
+              for (int h = 0; h < KMAXPT; h++) {
+                rc[h] += (double)qc[h];
+                rs[h] += (double)qs[h];
               }
             }
           };
@@ -407,6 +441,20 @@ __global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs(DevSys s, WsLayout L,
     sc[SC_P1] = acc_v1 + pk1 + c1;
     sc[SC_PT] = acc_v0 + acc_v1 + pkt + ct;
   }
+}
+
+__global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs(DevSys s, WsLayout L,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ ws,
+                                                     double* __restrict__ props) {
+  k_pairs_body<false>(s, L, x, ws, props);
+}
+
+__global__ __launch_bounds__(QMC_BLOCK, 5) void k_pairs_mix(DevSys s, WsLayout L,
+                                                         const double* __restrict__ x,
+                                                         double* __restrict__ ws,
+                                                         double* __restrict__ props) {
+  k_pairs_body<true>(s, L, x, ws, props);
 }
 
 // ---------------------------------------------------------------------------

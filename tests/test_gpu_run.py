@@ -228,6 +228,21 @@ def _blocks_rmc(text):
     return _blocks(text.replace("===>> rmc block", "===>> vmc block"))
 
 
+def _close_lines(ours, ref, rtol, what):
+    """Block lines: block and cumulative means to rtol relative, the
+    normalisation exactly, the error estimate (a difference of nearly equal
+    sums, rounding noise where the property is constant) to rtol of the
+    line's scale."""
+    for (a, na), (b, nb) in zip(ours, ref):
+        assert na == nb
+        scale = max(abs(b[0]), abs(b[1]), abs(b[2]), 1e-300)
+        for c in (0, 1):
+            assert abs(a[c] - b[c]) <= rtol * max(abs(b[c]), 1e-12 * scale), \
+                f"{what} {nb} column {c}: {a[c]!r} vs {b[c]!r}"
+        assert abs(a[2] - b[2]) <= rtol * scale, f"{what} {nb} error: {a[2]!r} vs {b[2]!r}"
+        assert a[3] == b[3], f"{what} {nb} norm"
+
+
 def _paths(nr, x0, seed):
     rng = np.random.default_rng(seed)
     out = {}
@@ -259,7 +274,7 @@ def test_rmc_one_reptile_reproduces_reference_driver_files(tmp_path):
     assert sorted(br) == sorted(bo) == [1, 2]
     for b in br:
         assert [n for _, n in br[b]] == [n for _, n in bo[b]]
-        _close([v for v, _ in bo[b]], [v for v, _ in br[b]], 1e-8, f"rmc block {b}")
+        _close_lines(bo[b], br[b], 1e-8, f"rmc block {b}")
     tr, to = _res(ref), _res(ours)
     assert tr[:2] == to[:2] == ["2", "rmc"] and len(tr) == len(to)
     _close([float(v) for v in to[2:-8]], [float(v) for v in tr[2:-8]], 1e-8, "restart sums")
@@ -277,7 +292,7 @@ def test_rmc_one_reptile_reproduces_reference_driver_files(tmp_path):
     out_r, out_o = _ref(ref), _ours(ours, 1)
     br, bo = _blocks_rmc(out_r), _blocks_rmc(out_o)
     assert sorted(br) == sorted(bo) == [3]
-    _close([v for v, _ in bo[3]], [v for v, _ in br[3]], 1e-8, "rmc block 3 after restart")
+    _close_lines(bo[3], br[3], 1e-8, "rmc block 3 after restart")
     assert _res(ref)[-8:] == _res(ours)[-8:]
 
 
