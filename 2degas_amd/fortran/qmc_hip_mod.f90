@@ -76,6 +76,13 @@ module qmc_hip_mod
        integer(c_int), value :: ng
      end function qmc_set_twist
 
+     integer(c_int) function qmc_eval_batch_device_noalloc(ctx, nw, d_x, d_props, d_g, d_h, &
+          d_grad2_ph, d_status, precision, stream) bind(C, name='qmc_eval_batch_device_noalloc')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, d_x, d_props, d_g, d_h, d_grad2_ph, d_status, stream
+       integer(c_int), value :: nw, precision
+     end function qmc_eval_batch_device_noalloc
+
      integer(c_int) function qmc_set_delta(ctx, delta) bind(C, name='qmc_set_delta')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

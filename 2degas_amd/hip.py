@@ -18,7 +18,7 @@ EXPORTS = ("qmc_ctx_create", "qmc_ctx_destroy", "qmc_ctx_info", "qmc_set_twist",
            "qmc_twist_gvec", "qmc_eval_batch", "qmc_eval_batch_device", "qmc_strerror",
            "qmc_last_error", "qmc_flops_per_eval", "qmc_timing", "qmc_timing_read",
            "qmc_vmc_start", "qmc_vmc_sweep", "qmc_vmc_status", "qmc_rmc_start",
-           "qmc_rmc_sweep", "qmc_set_delta")
+           "qmc_rmc_sweep", "qmc_set_delta", "qmc_eval_batch_device_noalloc")
 
 _lib = None
 
@@ -56,6 +56,7 @@ def lib():
         L.qmc_twist_gvec.argtypes = [dp, C.c_int, dp, C.c_int, C.c_int, C.c_int, dp]
         L.qmc_eval_batch.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp, ip]
         L.qmc_eval_batch_device.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, C.c_int, vp]
+        L.qmc_eval_batch_device_noalloc.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, C.c_int, vp]
         L.qmc_strerror.restype = C.c_char_p
         L.qmc_strerror.argtypes = [C.c_int]
         L.qmc_last_error.restype = C.c_char_p
@@ -164,19 +165,21 @@ class Context:
         return ms, n.value
 
     def eval_device(self, x, props, g, h, grad2, status=None, precision=abi.QMC_PREC_FP64,
-                    stream=None):
+                    stream=None, noalloc=False):
         """Device-pointer evaluation; arguments are torch tensors on the GPU
-        (or raw integer addresses).  Only enqueues on `stream`."""
+        (or raw integer addresses).  Only enqueues on `stream`.  noalloc
+        selects qmc_eval_batch_device_noalloc (workspace in the library's
+        fixed per-GPU scratch array)."""
         def ptr(t):
             if t is None:
                 return None
             return C.c_void_p(t if isinstance(t, int) else t.data_ptr())
         st = None if stream is None else C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
         nw = x.shape[0]
-        rc = lib().qmc_eval_batch_device(self.h, nw, ptr(x), ptr(props), ptr(g), ptr(h), ptr(grad2),
-                                         ptr(status), precision, st)
+        fn = lib().qmc_eval_batch_device_noalloc if noalloc else lib().qmc_eval_batch_device
+        rc = fn(self.h, nw, ptr(x), ptr(props), ptr(g), ptr(h), ptr(grad2), ptr(status), precision, st)
         if rc:
-            raise QmcError(rc, "qmc_eval_batch_device")
+            raise QmcError(rc, "qmc_eval_batch_device" + ("_noalloc" if noalloc else ""))
 
 
 class Walkers:

@@ -162,6 +162,27 @@ int qmc_eval_batch_device(qmc_ctx* ctx, int nw, const double* d_x,
                           void* stream);
 
 /*
+ * qmc_eval_batch_device without any allocation: no counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * evaluation's workspace lives in a file-scope device array of
+ * QMC_NOALLOC_SCRATCH_BYTES per GPU instead of context-owned memory; the
+ * caller passes inputs and outputs as for qmc_eval_batch_device.  The array
+ * is resolved once per device when a context is created on it, holds what
+ * the last call left, and is shared by every context on that GPU: calls
+ * that overlap on one GPU (several streams or threads) race.  Returns
+ * QMC_ERR_LIMIT when nw walkers of this context need more scratch than the
+ * limit (at N = 122, 256 KiB per walker: about 260 walkers), and does no
+ * per-kernel timing.
+ */
+#define QMC_NOALLOC_SCRATCH_BYTES (64ull << 20)
+#define QMC_NOALLOC_MAX_DEVICES 64
+int qmc_eval_batch_device_noalloc(qmc_ctx* ctx, int nw, const double* d_x,
+                                  double* d_props, double* d_g, double* d_h,
+                                  double* d_grad2_ph, int32_t* d_status, int precision,
+                                  void* stream);
+
+/*
  * Walker state of a batched VMC run, all in device memory and owned by the
  * caller (nw walkers; N = nptot).  Walker w plays the role of MPI rank w of
  * the reference: its own old configuration (/c_old/, ewald.h:49-52), age,
