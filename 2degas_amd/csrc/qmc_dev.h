@@ -171,6 +171,16 @@ __device__ __forceinline__ double rannyu(uint64_t& x) {
   return (double)x * 0x1p-48;
 }
 
+// a^n mod 2^48 for rannyu's multiplier: the stream n draws ahead is a^n x.
+__device__ __forceinline__ uint64_t lcg_pow(uint64_t n) {
+  uint64_t p = 1, q = 34522712143931ULL;
+  for (; n; n >>= 1) {
+    if (n & 1) p = (p * q) & ((1ULL << 48) - 1ULL);
+    q = (q * q) & ((1ULL << 48) - 1ULL);
+  }
+  return p;
+}
+
 enum { TK_U2 = 0, TK_U3 = 1, TK_B = 2, TK_V = 3 };
 
 // Element i of a 2- or 3-entry member array without a dynamic index (a

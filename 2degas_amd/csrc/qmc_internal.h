@@ -69,6 +69,10 @@ struct qmc_ctx {
   int vmc_nw = 0;
   double *d_xn = nullptr, *d_gn = nullptr, *d_hn = nullptr, *d_pn = nullptr, *d_wold = nullptr;
   int32_t *d_stn = nullptr, *d_sticky = nullptr;
+  // the VMC step as a captured graph, and what it was captured with
+  bool use_graph = true;
+  hipGraphExec_t vmc_exec = nullptr;
+  std::vector<uint64_t> vmc_key;
   // reptation scratch for rmc_nr reptiles: move draws and per-move sums
   int rmc_nr = 0;
   int32_t* d_rmv = nullptr;
@@ -83,6 +87,9 @@ int launch_eval(qmc_ctx* c, int nw, const double* d_x, double* d_props, double* 
                 double* d_g2, int32_t* d_status, hipStream_t st, double delta,
                 int precision = QMC_PREC_FP64);
 void free_vmc(qmc_ctx* c);
+void free_vmc_graph(qmc_ctx* c);
+// Workspace of the evaluation kernels for nw walkers (allocates when it grows).
+int ensure_ws(qmc_ctx* c, int nw);
 int ensure_vmc_scratch(qmc_ctx* c, int nw);
 void free_rmc(qmc_ctx* c);
 

@@ -58,16 +58,6 @@ __global__ void k_rmc_begin(RmcK k, const int32_t* __restrict__ ends, int64_t* _
   mv[3 * r + 2] = idir == 1 ? ends[2 * r + 1] : ends[2 * r];
 }
 
-// a^n mod 2^48 for rannyu's multiplier: the stream n draws ahead is a^n x.
-__device__ __forceinline__ uint64_t lcg_pow(uint64_t n) {
-  uint64_t p = 1, q = 34522712143931ULL;
-  for (; n; n >>= 1) {
-    if (n & 1) p = (p * q) & ((1ULL << 48) - 1ULL);
-    q = (q * q) & ((1ULL << 48) - 1ULL);
-  }
-  return p;
-}
-
 // Bead proposal idelta (reptation.f:1929-1955): x_new = chi - var g_j + x_j
 // from bead j, with wold = -sum chi^2/var, dold = sum drift^2/var and
 // lng = -sum (chi + drift)^2/var.  sc[r] = {wold, dold, lng}.

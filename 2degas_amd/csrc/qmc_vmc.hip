@@ -7,41 +7,67 @@
 // reptation.f:37-39).
 #include "qmc_internal.h"
 
+#include <cstring>
+
 // move, proposal half (reptation.f:4630-4646): for every moving type,
 // particle and dimension chi = cos(pi u1) sqrt(-2 var log u2),
 // x_new = chi - var g_old + x_old, w = -sum chi^2 / var.
-__global__ void k_vmc_propose(DevSys s, int nw, const double* __restrict__ x_old,
-                              const double* __restrict__ g_old, double* __restrict__ x_new,
-                              int64_t* __restrict__ rng, double* __restrict__ wold, double var0,
-                              double var1, int bm_order) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= nw) return;
-  uint64_t st = (uint64_t)rng[w];
+// One wavefront per walker, lanes over coordinates.  The reference draws two
+// uniforms per moved coordinate in coordinate order, so the coordinate with
+// ordinal o among the moved ones takes draws 2o+1 and 2o+2 of the walker's
+// stream: each lane jumps there with a^(2o+1).  The chi^2 terms go to LDS
+// and lane 0 adds them in the reference's order.
+__global__ __launch_bounds__(QMC_BLOCK) void k_vmc_propose(
+    DevSys s, int nw, const double* __restrict__ x_old, const double* __restrict__ g_old,
+    double* __restrict__ x_new, int64_t* __restrict__ rng, double* __restrict__ wold,
+    double var0, double var1, int bm_order) {
+  __shared__ double chi2[QMC_BLOCK / 64][2 * QMC_MAX_PARTICLES];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = blockIdx.x * (QMC_BLOCK / 64) + wv;
+  if (w >= nw) return;  // no barrier below: each wavefront works alone
+  const int N = s.nptot, n2 = 2 * N;
+  const double* xo = x_old + (int64_t)w * n2;
+  const double* go = g_old + (int64_t)w * n2;
+  double* xn = x_new + (int64_t)w * n2;
+  const uint64_t st0 = (uint64_t)rng[w];
   const double pi = 3.141592653589793;
-  const int N = s.nptot;
-  const double* xo = x_old + (int64_t)w * 2 * N;
-  const double* go = g_old + (int64_t)w * 2 * N;
-  double* xn = x_new + (int64_t)w * 2 * N;
-  double wo = 0.0;
-  for (int t = 0; t < s.ntypes; t++) {
-    const double var = t == 0 ? var0 : var1;
-    if (pick2(s.hbs2m, t) == 0.0) continue;
-    double wt = 0.0;
-    const int i0 = pick2(s.ipf, t), i1 = i0 + pick2(s.np, t);
-    for (int ip = i0; ip < i1; ip++)
-      for (int d = 0; d < 2; d++) {
-        double ua, ub;
-        if (bm_order == 0) { ua = rannyu(st); ub = rannyu(st); }
-        else { ub = rannyu(st); ua = rannyu(st); }
-        const double chi = cos(pi * ua) * sqrt(-2.0 * var * log(ub));
-        wt = wt - chi * chi;
-        const double drift = -var * go[2 * ip + d];
-        xn[2 * ip + d] = chi + drift + xo[2 * ip + d];
-      }
-    wo = wo + wt * (1.0 / var);
+  const uint64_t msk = (1ULL << 48) - 1ULL, a = 34522712143931ULL;
+  const bool two = s.ntypes > 1;
+  const bool mov0 = s.hbs2m[0] != 0.0, mov1 = two && s.hbs2m[1] != 0.0;
+  const int m0 = mov0 ? 2 * s.np[0] : 0;  // moved coordinates of type 0
+  const int nmov = m0 + (mov1 ? 2 * s.np[1] : 0);
+  double* c2 = chi2[wv];
+  for (int e = lane; e < n2; e += 64) {
+    const int t = two && e >= 2 * s.ipf[1] ? 1 : 0;
+    if (!(t ? mov1 : mov0)) {
+      xn[e] = xo[e];
+      continue;
+    }
+    const int o = t ? m0 + e - 2 * s.ipf[1] : e - 2 * s.ipf[0];
+    const double var = t ? var1 : var0;
+    const uint64_t x1 = (lcg_pow(2 * (uint64_t)o + 1) * st0) & msk, x2 = (a * x1) & msk;
+    const double u1 = (double)x1 * 0x1p-48, u2 = (double)x2 * 0x1p-48;
+    const double ua = bm_order == 0 ? u1 : u2, ub = bm_order == 0 ? u2 : u1;
+    const double chi = cos(pi * ua) * sqrt(-2.0 * var * log(ub));
+    c2[e] = chi * chi;
+    xn[e] = chi + (-var * go[e]) + xo[e];
   }
-  rng[w] = (int64_t)st;
-  wold[w] = wo;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane == 0) {
+    double wo = 0.0;
+    for (int t = 0; t < s.ntypes; t++) {
+      if (!(t ? mov1 : mov0)) continue;
+      const double var = t == 0 ? var0 : var1;
+      const int e0 = 2 * pick2(s.ipf, t), e1 = e0 + 2 * pick2(s.np, t);
+      double wt = 0.0;
+      for (int e = e0; e < e1; e++) wt = wt - c2[e];
+      wo = wo + wt * (1.0 / var);
+    }
+    wold[w] = wo;
+    rng[w] = (int64_t)((lcg_pow(2 * (uint64_t)nmov) * st0) & msk);
+  }
 }
 
 // move, acceptance half (reptation.f:4648-4678) + metropolis_test
@@ -139,6 +165,7 @@ __global__ void k_vmc_start(int nw, int np_, int jacc, double* __restrict__ prop
 }
 
 void free_vmc(qmc_ctx* c) {
+  free_vmc_graph(c);
   (void)hipFree(c->d_xn);
   (void)hipFree(c->d_gn);
   (void)hipFree(c->d_hn);
@@ -193,6 +220,30 @@ extern "C" int qmc_vmc_start(qmc_ctx* c, const qmc_walkers* w, void* stream) {
   return QMC_OK;
 }
 
+// One VMC step for every walker: proposal, evaluation, acceptance.
+static int vmc_step(qmc_ctx* c, const qmc_walkers* w, hipStream_t st, double delta, double var0,
+                    double var1, double mage, int bm_order, double* tp, double* ta) {
+  const DevSys& s = c->s;
+  const int nw = w->nw;
+  const int wpb = QMC_BLOCK / 64;
+  hipLaunchKernelGGL(k_vmc_propose, dim3((nw + wpb - 1) / wpb), dim3(QMC_BLOCK), 0, st, s, nw,
+                     w->x, w->g, c->d_xn, w->rng, c->d_wold, var0, var1, bm_order);
+  // the fixed-phase scratch is shared by proposals, as the COMMON is
+  int rc = launch_eval(c, nw, c->d_xn, c->d_pn, c->d_gn, c->d_hn, w->grad2_ph, c->d_stn, st, delta);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_vmc_accept, dim3((nw + wpb - 1) / wpb), dim3(QMC_BLOCK), 0, st, s, nw,
+                     var0, var1, mage, w->x, w->g, w->h, w->props, w->age, c->d_xn, c->d_gn,
+                     c->d_hn, c->d_pn, w->rng, c->d_wold, c->d_stn, c->d_sticky, w->blk, tp, ta,
+                     c->info.jacc - 1, c->info.jltf - 1);
+  return QMC_OK;
+}
+
+void free_vmc_graph(qmc_ctx* c) {
+  if (c->vmc_exec) (void)hipGraphExecDestroy(c->vmc_exec);
+  c->vmc_exec = nullptr;
+  c->vmc_key.clear();
+}
+
 extern "C" int qmc_vmc_sweep(qmc_ctx* c, const qmc_walkers* w, int nsteps, double delta,
                              double mage, int bm_order, double* trace_p, double* trace_age,
                              void* stream) {
@@ -204,6 +255,8 @@ extern "C" int qmc_vmc_sweep(qmc_ctx* c, const qmc_walkers* w, int nsteps, doubl
   hipStream_t st = (hipStream_t)stream;
   rc = ensure_vmc_scratch(c, w->nw);
   if (rc) return rc;
+  rc = ensure_ws(c, w->nw);
+  if (rc) return rc;
   const DevSys& s = c->s;
   // the vmc record sets the COMMON delta before sigma (reptation.f:740-746)
   c->delta = delta;
@@ -211,21 +264,53 @@ extern "C" int qmc_vmc_sweep(qmc_ctx* c, const qmc_walkers* w, int nsteps, doubl
   const double var0 = 2.0 * s.hbs2m[0] * delta;
   const double var1 = s.ntypes > 1 ? 2.0 * s.hbs2m[1] * delta : 1.0;
   const int nw = w->nw;
-  for (int k = 0; k < nsteps; k++) {
-    hipLaunchKernelGGL(k_vmc_propose, dim3((nw + 63) / 64), dim3(64), 0, st, s, nw, w->x, w->g,
-                       c->d_xn, w->rng, c->d_wold, var0, var1, bm_order);
-    // the fixed-phase scratch is shared by proposals, as the COMMON is
-    rc = launch_eval(c, nw, c->d_xn, c->d_pn, c->d_gn, c->d_hn, w->grad2_ph, c->d_stn, st, delta);
-    if (rc) return rc;
-    const int wpb = QMC_BLOCK / 64;
-    hipLaunchKernelGGL(k_vmc_accept, dim3((nw + wpb - 1) / wpb), dim3(QMC_BLOCK), 0, st, s, nw,
-                       var0, var1, mage, w->x, w->g, w->h, w->props, w->age, c->d_xn, c->d_gn,
-                       c->d_hn, c->d_pn, w->rng, c->d_wold, c->d_stn, c->d_sticky, w->blk,
-                       trace_p ? trace_p + (int64_t)k * nw : nullptr,
-                       trace_age ? trace_age + (int64_t)k * nw : nullptr, c->info.jacc - 1,
-                       c->info.jltf - 1);
+  const bool graph = c->use_graph && !trace_p && !trace_age && !c->timing && nsteps > 1;
+  if (!graph) {
+    for (int k = 0; k < nsteps; k++) {
+      rc = vmc_step(c, w, st, delta, var0, var1, mage, bm_order,
+                    trace_p ? trace_p + (int64_t)k * nw : nullptr,
+                    trace_age ? trace_age + (int64_t)k * nw : nullptr);
+      if (rc) return rc;
+    }
+    HIPCHK(hipGetLastError());
+    return QMC_OK;
   }
-  HIPCHK(hipGetLastError());
+  // The step as a captured graph, replayed nsteps times: rebuilt whenever
+  // anything its kernels were given changes (buffers, step parameters, the
+  // g-vectors).  Captured on the context's own stream, launched on st.
+  double dk[4] = {var0, var1, mage, delta};
+  std::vector<uint64_t> key = {(uint64_t)nw, (uint64_t)(uintptr_t)w->x, (uint64_t)(uintptr_t)w->g,
+                               (uint64_t)(uintptr_t)w->h, (uint64_t)(uintptr_t)w->props,
+                               (uint64_t)(uintptr_t)w->age, (uint64_t)(uintptr_t)w->rng,
+                               (uint64_t)(uintptr_t)w->blk, (uint64_t)(uintptr_t)w->grad2_ph,
+                               (uint64_t)(uintptr_t)c->d_xn, (uint64_t)(uintptr_t)c->d_ws,
+                               (uint64_t)(uintptr_t)c->d_F, (uint64_t)(uintptr_t)c->d_A,
+                               (uint64_t)(uintptr_t)s.gvec, (uint64_t)s.ng, (uint64_t)bm_order};
+  for (double v : dk) {
+    uint64_t b;
+    memcpy(&b, &v, sizeof b);
+    key.push_back(b);
+  }
+  if (!c->vmc_exec || key != c->vmc_key) {
+    free_vmc_graph(c);
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = vmc_step(c, w, c->stream, delta, var0, var1, mage, bm_order, nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    const hipError_t e2 = hipGraphInstantiate(&c->vmc_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+      c->vmc_exec = nullptr;
+      return hip_fail(e2, "hipGraphInstantiate");
+    }
+    c->vmc_key = key;
+  }
+  for (int k = 0; k < nsteps; k++) HIPCHK(hipGraphLaunch(c->vmc_exec, st));
   return QMC_OK;
 }
 

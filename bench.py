@@ -139,6 +139,10 @@ def main():
                     help="served workload instead: Poisson arrivals of independent VMC jobs "
                          "(N in {26,58,74,122} or polarized {37,49,61}, rs in [1,20], +-10% "
                          "parameters) run to 1e-4 Ha (2e-4 Ry) per electron")
+    ap.add_argument("--vmc", action="store_true",
+                    help="VMC steps of the plain N=26 workload instead: 13+13, rs=5, u2 only, "
+                         "zero backflow, 60 k-shells, 256 walkers; the step replayed as a "
+                         "captured hipGraph and launched directly (QMC_NO_GRAPH)")
     ap.add_argument("--jobs", type=int, default=40)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -164,6 +168,8 @@ def main():
     dg = importlib.import_module("2degas_amd.deckgen")
     if args.rqmc:
         return bench_rqmc(args, hip, deck, dg)
+    if args.vmc:
+        return bench_vmc(args, hip, deck, dg)
     if args.serve:
         return bench_serve(args)
     tmp = tempfile.mkdtemp(prefix="degas_bench_")
@@ -284,6 +290,65 @@ def bench_mixed(args, ctx, xt, props, g, h, g2, st, stream, ref64, value64):
             "error_vs_fp64": {"elocal": q(e_el), "ltf": q(e_ltf), "drift": q(e_dr)},
             "target_3x_met": bool(v / value64 >= 3.0),
             "target_1e-6_met": bool(np.max(e_el) <= 1e-6)}
+
+
+def bench_vmc(args, hip, deck, dg):
+    """VMC steps/s (move + evaluation + Metropolis test + block sums of every
+    walker) on the plain N=26 workload with 256 walkers, the step captured
+    as a hipGraph and replayed, beside the same steps launched directly."""
+    import numpy as np
+    import torch
+    abi = importlib.import_module("2degas_amd.abi")
+    nw = 256
+    tmp = tempfile.mkdtemp(prefix="degas_vmc_")
+    deckdir = os.path.join(tmp, "deck")
+    dg.make_deck(deckdir, "c26", 13, 13, 5.0, 60, u3=False, backflow="zero")
+    d = deck.read_deck(deckdir)
+    x = dg.lattice_configs(nw, d.nptot, d.el[0], 4321)
+    sts = [abi.limbs_to_state([3535, 3467, 3262, (2 * (1549 + k) + 1) % 4096]) for k in range(nw)]
+    res = {}
+    for mode in ("graph", "direct"):
+        if mode == "direct":
+            os.environ["QMC_NO_GRAPH"] = "1"
+        try:
+            ctx = hip.Context(d, twist=(1, 1), device=0)
+        finally:
+            os.environ.pop("QMC_NO_GRAPH", None)
+        wk = hip.Walkers(ctx, x, sts)
+        wk.start(delta=0.2)
+        wk.sweep(max(args.warmup, 2), 0.2)
+        torch.cuda.synchronize()
+        nsteps = 200
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            wk.sweep(nsteps, 0.2)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[mode] = args.steps * nsteps / dt
+        acc = float((wk.age == 0).double().mean())
+        wk = None
+        ctx.close()
+    out = {
+        "metric": "VMC steps/s (plain N=26, 256 walkers)",
+        "value": res["graph"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps * 200,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / res["graph"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "VMC N=26 (13+13) rs=5 u2 only, zero backflow, 60 k-shells (232 k), "
+                               "256 walkers, delta 0.2", "walkers": nw},
+        "walker_moves_per_s": res["graph"] * nw,
+        "direct_launch_steps_per_s": res["direct"],
+        "graph_speedup": res["graph"] / res["direct"],
+        "acceptance_last_step": acc,
+    }
+    print(json.dumps(out))
 
 
 def bench_serve(args):

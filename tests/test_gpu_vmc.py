@@ -116,3 +116,43 @@ def test_vmc_other_draw_order_matches_oracle(oracle_mod):
     print(f"   moves {nw * nsteps} mismatches {mism} max rel p {worst:.2e}", flush=True)
     assert mism == 0
     assert worst < 1e-8
+
+
+def test_vmc_graph_replays_match_direct_launches():
+    """qmc_vmc_sweep without a trace replays the step as a captured hipGraph;
+    a context made with QMC_NO_GRAPH launches every kernel directly.  From the
+    same walkers and streams, each replayed sweep leaves exactly the state of
+    the direct one (positions, drifts, properties, ages, generator states,
+    block sums), with other eager work between the replays and after a change
+    of time step (a new capture)."""
+    print("\n[gpu] vmc graph replays vs direct launches", flush=True)
+    import os
+    import torch
+    hip = pkg("hip")
+    abi = pkg("abi")
+    dg = pkg("deckgen")
+    d = load_deck("c26")
+    x = dg.lattice_configs(64, d.nptot, d.el[0], 31)
+    sts = [abi.limbs_to_state([3535, 3467, 3262, (2 * k + 1) % 4096]) for k in range(64)]
+    ctx_g = hip.Context(d, twist=(1, 1))
+    os.environ["QMC_NO_GRAPH"] = "1"
+    try:
+        ctx_d = hip.Context(d, twist=(1, 1))
+    finally:
+        del os.environ["QMC_NO_GRAPH"]
+    wg, wd = hip.Walkers(ctx_g, x, sts), hip.Walkers(ctx_d, x, sts)
+    wg.start(delta=0.2)
+    wd.start(delta=0.2)
+    other = torch.tensor(dg.random_configs(16, d.nptot, d.el[0], 5), device="cuda:0")
+    for rep, delta in enumerate((0.2, 0.2, 0.2, 0.1, 0.1)):
+        wg.sweep(7, delta)
+        wd.sweep(7, delta)
+        torch.cuda.synchronize()
+        for name in ("x", "g", "h", "props", "age", "rng", "blk", "grad2"):
+            a, b = getattr(wg, name).cpu().numpy(), getattr(wd, name).cpu().numpy()
+            assert np.array_equal(a, b), f"replay {rep}: {name} differs"
+        # eager work on the graph context between replays
+        ctx_g.eval(other.cpu().numpy())
+    assert np.all(wg.status() == 0)
+    print(f"   5 sweeps of 7 steps identical, acceptance "
+          f"{float((wg.age == 0).double().mean()):.2f}", flush=True)
