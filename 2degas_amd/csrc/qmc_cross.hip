@@ -221,6 +221,7 @@ __global__ __launch_bounds__(XC_THREADS, 4) void k_cross(DevSys s, WsLayout L, c
   // walkers and spins in the reverse of the order in which k_gjm wrote their
   // F, so the first workgroups find theirs still in the memory-side cache
   const int w = gridDim.x - 1 - blockIdx.x, sp = gridDim.y - 1 - blockIdx.y;
+  if (s.nact && w >= *s.nact) return;
   const int m = pick2(s.np, sp);
   if (m == 0) return;
   const int N = s.nptot, p0 = pick2(s.ipf, sp);

@@ -75,6 +75,9 @@ struct DevSys {
   // orbitals
   const double* gvec;      // [ng][2]
   int ng, has_det;
+  // walkers evaluated: the first *nact of the batch when not null (the
+  // reptation sweep evaluates only the reptiles that still move a bead)
+  const int* nact;
 };
 
 // Per-walker workspace (doubles) and per-spin determinant buffers.

@@ -29,6 +29,7 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_fgrad(DevSys s, WsLayout L,
   extern __shared__ __attribute__((aligned(16))) double sm[];
   __shared__ double red[QMC_BLOCK / 64];
   const int w = blockIdx.x, sp = blockIdx.y;
+  if (s.nact && w >= *s.nact) return;
   const int m = pick2(s.np, sp);
   if (m == 0) return;
   const int N = s.nptot, p0 = pick2(s.ipf, sp);

@@ -144,6 +144,7 @@ __device__ __forceinline__ void gjm_run(DevSys s, WsLayout L, double* __restrict
   extern __shared__ __attribute__((aligned(16))) unsigned char gm_lds[];
   GmShared& sh = *reinterpret_cast<GmShared*>(gm_lds);
   const int w = blockIdx.x, sp = blockIdx.y;
+  if (s.nact && w >= *s.nact) return;
   const int m = pick2(s.np, sp);
   if (m == 0) return;
   const int p0 = pick2(s.ipf, sp);

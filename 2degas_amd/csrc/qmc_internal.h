@@ -77,15 +77,17 @@ struct qmc_ctx {
   int rmc_nr = 0;
   int32_t* d_rmv = nullptr;
   double* d_rsc = nullptr;
+  int32_t* d_rslot = nullptr;  // [nr] evaluation slot of each reptile, [mdelta] counts after it
 };
 
 // Queue one batched evaluation (k_pairs, k_gjm or k_gjm_part and k_fgrad, or
 // k_det, then k_cross, k_final) on st.
 // delta is the time step the drift cutoff of kinetic uses (reptation.f:2755);
-// precision QMC_PREC_MIXED selects the mixed-precision pair kernel.
+// precision QMC_PREC_MIXED selects the mixed-precision pair kernel; nact (a
+// device pointer, or null) limits the evaluation to the first *nact walkers.
 int launch_eval(qmc_ctx* c, int nw, const double* d_x, double* d_props, double* d_g, double* d_h,
                 double* d_g2, int32_t* d_status, hipStream_t st, double delta,
-                int precision = QMC_PREC_FP64);
+                int precision = QMC_PREC_FP64, const int* nact = nullptr);
 void free_vmc(qmc_ctx* c);
 void free_vmc_graph(qmc_ctx* c);
 // Workspace of the evaluation kernels for nw walkers (allocates when it grows).

@@ -73,6 +73,7 @@ __device__ __forceinline__ void k_pairs_body(const DevSys& s, const WsLayout& L,
                                              double* __restrict__ props) {
   extern __shared__ __attribute__((aligned(16))) double sm[];
   const int w = blockIdx.x;
+  if (s.nact && w >= *s.nact) return;
   const int N = s.nptot;
   double* xs = sm;
   double* Gs = sm + 2 * N;
@@ -475,6 +476,7 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_det(DevSys s, WsLayout L,
   __shared__ zd s_pv, s_akk;
   __shared__ double red[QMC_BLOCK / 64];
   const int w = blockIdx.x, sp = blockIdx.y;
+  if (s.nact && w >= *s.nact) return;
   const int m = pick2(s.np, sp);
   if (m == 0) return;
   const int N = s.nptot, p0 = pick2(s.ipf, sp);
@@ -831,7 +833,7 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_final(DevSys s, WsLayout L, doubl
                                                      double delta) {
   const int w = blockIdx.x * (QMC_BLOCK / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (w >= nw) return;
+  if (w >= nw || (s.nact && w >= *s.nact)) return;
   const int N = s.nptot;
   double* wsw = ws + (int64_t)w * L.per_walker;
   const double* sc = wsw + L.off_sc;

@@ -8,15 +8,16 @@
 // drift-diffusion, each evaluated and passed through a Langevin test, and
 // then accepts or rejects the whole slide with the path weight
 // (reptation_metropolis_test, reptation.f:1865-1895).  On the device a move is
-// k_rmc_begin, then for idelta = 1..mdelta a proposal kernel, one batched
-// evaluation of every reptile and a bead kernel, then k_rmc_path and
-// k_rmc_finish.  A
-// reptile whose ndelta is below idelta carries its end bead through the
-// evaluation unchanged and draws nothing.
+// k_rmc_begin (directions and ndelta), k_rmc_slots (evaluation slots ordered
+// by ndelta), then for idelta = 1..mdelta a proposal kernel, one batched
+// evaluation of the reptiles that move bead idelta (the first slots) and a
+// bead kernel, then k_rmc_path and k_rmc_finish.  A reptile whose ndelta is
+// below idelta is neither evaluated nor draws.
 #include "qmc_internal.h"
 
 enum { RW_BRA = 0, RW_DTP = 1, RW_ITP = 2, RW_LNG = 3, RW_NDS = 4, RW_N = 5 };
 #define RMC_TR 8  // fixed trace fields before the per-bead probabilities
+#define RMC_MAX_MDELTA 64  // largest number of beads one move may grow
 
 struct RmcK {
   int nr, ntau, nbuf, N, NP, ntypes;
@@ -58,6 +59,36 @@ __global__ void k_rmc_begin(RmcK k, const int32_t* __restrict__ ends, int64_t* _
   mv[3 * r + 2] = idir == 1 ? ends[2 * r + 1] : ends[2 * r];
 }
 
+// Evaluation slots of a move: the reptiles ordered by ndelta, largest first,
+// so that those moving bead idelta or more hold slots 0..nact[idelta-1]-1
+// and each per-bead evaluation covers only them.  One workgroup; the order
+// inside a group of equal ndelta is whatever the LDS atomics give (a
+// reptile's evaluation does not depend on its slot).
+#define RMC_SLOT_THREADS 1024
+__global__ __launch_bounds__(RMC_SLOT_THREADS) void k_rmc_slots(RmcK k, const int32_t* __restrict__ mv,
+                                                              int32_t* __restrict__ slot,
+                                                              int32_t* __restrict__ nact) {
+  __shared__ int cnt[RMC_MAX_MDELTA + 2], base[RMC_MAX_MDELTA + 2], fill[RMC_MAX_MDELTA + 2];
+  const int tid = threadIdx.x;
+  for (int b = tid; b < RMC_MAX_MDELTA + 2; b += RMC_SLOT_THREADS) cnt[b] = base[b] = fill[b] = 0;
+  __syncthreads();
+  for (int r = tid; r < k.nr; r += RMC_SLOT_THREADS) atomicAdd(&cnt[mv[3 * r + 1]], 1);
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int b = k.mdelta; b >= 1; b--) {
+      base[b] = acc;
+      acc += cnt[b];
+      nact[b - 1] = acc;  // reptiles with ndelta >= b
+    }
+  }
+  __syncthreads();
+  for (int r = tid; r < k.nr; r += RMC_SLOT_THREADS) {
+    const int b = mv[3 * r + 1];
+    slot[r] = base[b] + atomicAdd(&fill[b], 1);
+  }
+}
+
 // Bead proposal idelta (reptation.f:1929-1955): x_new = chi - var g_j + x_j
 // from bead j, with wold = -sum chi^2/var, dold = sum drift^2/var and
 // lng = -sum (chi + drift)^2/var.  sc[r] = {wold, dold, lng}.
@@ -68,8 +99,9 @@ __global__ void k_rmc_begin(RmcK k, const int32_t* __restrict__ ends, int64_t* _
 // lanes add them in the reference's order.
 __global__ __launch_bounds__(QMC_BLOCK) void k_rmc_propose(
     RmcK k, int idelta, const double* __restrict__ X, const double* __restrict__ G,
-    const int32_t* __restrict__ mv, int64_t* __restrict__ rng, double* __restrict__ xn,
-    double* __restrict__ sc, const double* __restrict__ g2, double* __restrict__ g2n) {
+    const int32_t* __restrict__ mv, const int32_t* __restrict__ slot, int64_t* __restrict__ rng,
+    double* __restrict__ xn, double* __restrict__ sc, const double* __restrict__ g2,
+    double* __restrict__ g2n) {
   extern __shared__ double rp_sm[];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = blockIdx.x * (QMC_BLOCK / 64) + wv;
@@ -78,12 +110,11 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_rmc_propose(
   const int idir = live ? mv[3 * r] : 0, nd = live ? mv[3 * r + 1] : 0, js = live ? mv[3 * r + 2] : 0;
   const bool prop = live && idelta <= nd;
   const int N = k.N, n2 = 2 * N;
-  double* xo = xn + (int64_t)r * n2;
-  if (live && lane == 0) g2n[r] = g2[r];
-  if (live && !prop) {
-    const double* xe = X + ((int64_t)r * k.nbuf + js) * n2;
-    for (int e = lane; e < n2; e += 64) xo[e] = xe[e];
-  }
+  // the proposal goes to the reptile's evaluation slot: the reptiles still
+  // moving a bead hold the first slots (k_rmc_slots)
+  const int sl = prop ? slot[r] : 0;
+  double* xo = xn + (int64_t)sl * n2;
+  if (prop && lane == 0) g2n[sl] = g2[r];
   const int j = rslot(js + idir * (idelta - 1), k.nbuf);
   const double* xj = X + ((int64_t)r * k.nbuf + j) * n2;
   const double* gj = G + ((int64_t)r * k.nbuf + j) * n2;
@@ -126,7 +157,7 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_rmc_propose(
         for (int e = e0; e < e1; e++) v = v - tq[e];
       acc = acc + v * (t ? k.vari[1] : k.vari[0]);
     }
-    sc[3 * r + (lane == 0 ? 0 : (lane == 1 ? 1 : 2))] = acc;
+    sc[3 * sl + (lane == 0 ? 0 : (lane == 1 ? 1 : 2))] = acc;
     if (lane == 0) rng[r] = (int64_t)((lcg_pow(2 * (uint64_t)nmov) * st0) & msk);
   }
 }
@@ -136,7 +167,8 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_rmc_propose(
 __global__ __launch_bounds__(QMC_BLOCK) void k_rmc_step(
     RmcK k, int idelta, double* __restrict__ X, double* __restrict__ G, double* __restrict__ P,
     double* __restrict__ AGE, double* __restrict__ W, const int32_t* __restrict__ mv,
-    int64_t* __restrict__ rng, const double* __restrict__ xn, const double* __restrict__ gn,
+    const int32_t* __restrict__ slot, int64_t* __restrict__ rng, const double* __restrict__ xn,
+    const double* __restrict__ gn,
     const double* __restrict__ pn, const double* __restrict__ sc, double* __restrict__ g2,
     const double* __restrict__ g2n, const int32_t* __restrict__ stn, int32_t* __restrict__ sticky,
     double* __restrict__ trace) {
@@ -158,9 +190,10 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_rmc_step(
   double* xq = X + ((int64_t)r * nbuf + jn) * 2 * N;
   double* gq = G + ((int64_t)r * nbuf + jn) * 2 * N;
   double* pq = P + ((int64_t)r * nbuf + jn) * NP;
-  const double* xr = xn + (int64_t)r * 2 * N;
-  const double* gr = gn + (int64_t)r * 2 * N;
-  const double* pr = pn + (int64_t)r * NP;
+  const int sl = slot[r];  // the reptile's evaluation slot
+  const double* xr = xn + (int64_t)sl * 2 * N;
+  const double* gr = gn + (int64_t)sl * 2 * N;
+  const double* pr = pn + (int64_t)sl * NP;
   // reverse-move weight (reptation.f:1958-1970)
   double wt[2] = {0.0, 0.0};
   for (int ip = lane; ip < N; ip += 64) {
@@ -180,8 +213,8 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_rmc_step(
     double* Wr = W + (int64_t)r * nbuf * RW_N;
     double* wj = Wr + j * RW_N;
     double* wq = Wr + jn * RW_N;
-    const double wold = sc[3 * r], dold = sc[3 * r + 1];
-    double lng = sc[3 * r + 2];
+    const double wold = sc[3 * sl], dold = sc[3 * sl + 1];
+    double lng = sc[3 * sl + 2];
     double wnew = 0.0;
     for (int t = 0; t < k.ntypes; t++)
       if (k.mov[t]) wnew = wnew + wt[t] * k.vari[t];
@@ -223,8 +256,8 @@ __global__ __launch_bounds__(QMC_BLOCK) void k_rmc_step(
       }
       AGE[(int64_t)r * nbuf + jn] = 0.0;
     }
-    g2[r] = g2n[r];
-    sticky[r] |= stn[r];
+    g2[r] = g2n[sl];
+    sticky[r] |= stn[sl];
     if (trace) trace[(int64_t)r * rowlen + RMC_TR + idelta - 1] = p;
   }
   acc = __shfl(acc, 0, 64);
@@ -469,12 +502,16 @@ static int ensure_rmc(qmc_ctx* c, int nr) {
   if (nr <= c->rmc_nr) return QMC_OK;
   (void)hipFree(c->d_rmv);
   (void)hipFree(c->d_rsc);
+  (void)hipFree(c->d_rslot);
   c->d_rmv = nullptr;
   c->d_rsc = nullptr;
+  c->d_rslot = nullptr;
   const size_t NP = c->info.n_props;
   HIPCHK(hipMalloc(&c->d_rmv, sizeof(int32_t) * 3 * nr));
   // sc[3], g2n[1], path averages [NP]
   HIPCHK(hipMalloc(&c->d_rsc, sizeof(double) * (4 + NP) * nr));
+  // evaluation slot per reptile, then the counts per bead of a move
+  HIPCHK(hipMalloc(&c->d_rslot, sizeof(int32_t) * (nr + RMC_MAX_MDELTA)));
   c->rmc_nr = nr;
   return QMC_OK;
 }
@@ -482,8 +519,10 @@ static int ensure_rmc(qmc_ctx* c, int nr) {
 void free_rmc(qmc_ctx* c) {
   (void)hipFree(c->d_rmv);
   (void)hipFree(c->d_rsc);
+  (void)hipFree(c->d_rslot);
   c->d_rmv = nullptr;
   c->d_rsc = nullptr;
+  c->d_rslot = nullptr;
   c->rmc_nr = 0;
 }
 
@@ -496,6 +535,7 @@ static int rmc_setup(qmc_ctx* c, const qmc_reptiles* r, const qmc_rmc_params* pr
   if (!(prm->delta > 0.0)) return arg_fail("delta must be positive");
   if (prm->mdelta < 1 || r->ntau < 2 || r->nbuf != r->ntau + prm->mdelta)
     return arg_fail("reptile needs ntau >= 2, mdelta >= 1 and nbuf = ntau + mdelta");
+  if (prm->mdelta > RMC_MAX_MDELTA) return arg_fail("mdelta above 64", QMC_ERR_LIMIT);
   if (prm->ntauskip < 0 || r->ntau - 2 * prm->ntauskip < 1)
     return arg_fail("ntauskip leaves no bead for the path averages");
   for (int t = 0; t < s.ntypes; t++)
@@ -604,15 +644,19 @@ extern "C" int qmc_rmc_sweep(qmc_ctx* c, const qmc_reptiles* r, const qmc_rmc_pa
     double* tr = trace ? trace + (int64_t)s * nr * rowlen : nullptr;
     hipLaunchKernelGGL(k_rmc_begin, dim3((nr + 63) / 64), dim3(64), 0, st, k, r->ends, r->rng,
                        c->d_rmv);
+    hipLaunchKernelGGL(k_rmc_slots, dim3(1), dim3(RMC_SLOT_THREADS), 0, st, k, c->d_rmv, c->d_rslot,
+                       c->d_rslot + nr);
     for (int id = 1; id <= prm->mdelta; id++) {
       hipLaunchKernelGGL(k_rmc_propose, dim3((nr + wpb - 1) / wpb), dim3(QMC_BLOCK),
                          sizeof(double) * 3 * 2 * k.N * wpb, st, k, id, r->x, r->g, c->d_rmv,
-                         r->rng, c->d_xn, sc, r->grad2_ph, g2n);
-      rc = launch_eval(c, nr, c->d_xn, c->d_pn, c->d_gn, c->d_hn, g2n, c->d_stn, st, prm->delta);
+                         c->d_rslot, r->rng, c->d_xn, sc, r->grad2_ph, g2n);
+      // only the reptiles moving bead id are evaluated (slots 0..nact-1)
+      rc = launch_eval(c, nr, c->d_xn, c->d_pn, c->d_gn, c->d_hn, g2n, c->d_stn, st, prm->delta,
+                       QMC_PREC_FP64, c->d_rslot + nr + id - 1);
       if (rc) return rc;
       hipLaunchKernelGGL(k_rmc_step, dim3((nr + wpb - 1) / wpb), dim3(QMC_BLOCK), 0, st, k, id,
-                         r->x, r->g, r->props, r->age, r->w, c->d_rmv, r->rng, c->d_xn, c->d_gn,
-                         c->d_pn, sc, r->grad2_ph, g2n, c->d_stn, c->d_sticky, tr);
+                         r->x, r->g, r->props, r->age, r->w, c->d_rmv, c->d_rslot, r->rng, c->d_xn,
+                         c->d_gn, c->d_pn, sc, r->grad2_ph, g2n, c->d_stn, c->d_sticky, tr);
     }
     hipLaunchKernelGGL(k_rmc_path, dim3(nr, (k.NP + 63) / 64), dim3(64), 0, st, k, r->props,
                        r->ends, c->d_rmv, ppn);

@@ -433,7 +433,8 @@ def bench_rqmc(args, hip, deck, dg):
         "config": {"workload": f"RQMC N={N} (29+29) rs=5 u2+u3+backflow, {nr} reptiles x {ntau} beads, "
                                f"mdelta {md}, delta 0.01 (tau 0.02 Ha^-1)",
                    "reptiles": nr, "ntau": ntau, "mdelta": md},
-        "bead_evals_per_s": nr * md * args.steps / dt,
+        # ndelta is uniform in 1..mdelta and only moving beads are evaluated
+        "bead_evals_per_s_expected": nr * (md + 1) / 2 * args.steps / dt,
         "start_s": t_start,
         "status_nonzero": int(np.sum(st != 0)),
     }
