@@ -222,9 +222,10 @@ __device__ __forceinline__ void gjm_run(DevSys s, WsLayout L, double* __restrict
       __builtin_amdgcn_s_setprio(0);
     }
     __syncthreads();
-    int pb[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) pb[j] = j < nb ? __builtin_amdgcn_readfirstlane(sh.prow[k0 + j]) : 0;
+    // the block's 4 pivot rows in one LDS read (entries beyond nb unused)
+    const int4 pr4 = *reinterpret_cast<const int4*>(&sh.prow[k0]);
+    const int pb[4] = {__builtin_amdgcn_readfirstlane(pr4.x), __builtin_amdgcn_readfirstlane(pr4.y),
+                       __builtin_amdgcn_readfirstlane(pr4.z), __builtin_amdgcn_readfirstlane(pr4.w)};
     // the panel columns become e_{p_j}
     const bool panel_tile = ct == (k0 >> 4);
     const int jj = lc - (k0 & 15);
