@@ -73,6 +73,11 @@ class QmcInfo(C.Structure):
         ("jun", C.c_int32 * 2),
         ("ng", C.c_int32),
         ("has_det", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("dev_bytes", C.c_uint64),
+        ("eval_bytes_per_walker", C.c_uint64),
+        ("vmc_bytes_per_walker", C.c_uint64),
+        ("stage_bytes_per_walker", C.c_uint64),
     ]
 
     def as_dict(self):

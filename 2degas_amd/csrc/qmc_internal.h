@@ -17,6 +17,8 @@ inline int hip_fail(hipError_t e, const char* what) {
   g_last_error = buf;
   return QMC_ERR_HIP;
 }
+#define RMC_MAX_MDELTA 64  // largest number of beads one reptation move may grow
+
 #define HIPCHK(call)                                       \
   do {                                                     \
     hipError_t e_ = (call);                                \
@@ -42,6 +44,8 @@ struct qmc_ctx {
   TpInfo* d_tpi = nullptr;
   // workspace sized for ws_nw walkers
   int ws_nw = 0;
+  // device bytes of the uploaded tables (not counting the g-vectors)
+  size_t table_bytes = 0;
   double* d_ws = nullptr;
   double* d_F = nullptr;
   double* d_A = nullptr;

@@ -17,7 +17,6 @@
 
 enum { RW_BRA = 0, RW_DTP = 1, RW_ITP = 2, RW_LNG = 3, RW_NDS = 4, RW_N = 5 };
 #define RMC_TR 8  // fixed trace fields before the per-bead probabilities
-#define RMC_MAX_MDELTA 64  // largest number of beads one move may grow
 
 struct RmcK {
   int nr, ntau, nbuf, N, NP, ntypes;

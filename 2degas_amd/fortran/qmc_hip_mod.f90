@@ -31,7 +31,9 @@ module qmc_hip_mod
      integer(c_int32_t) :: nptot, npnorm, n_scalar_props, n_props
      integer(c_int32_t) :: jetot, jltf, jacc, jpot0, jkin0, je2, jgg, jefp, jrhok
      integer(c_int32_t) :: jpot(2), jkin(2), jun(2)
-     integer(c_int32_t) :: ng, has_det
+     integer(c_int32_t) :: ng, has_det, reserved0
+     integer(c_int64_t) :: dev_bytes, eval_bytes_per_walker, vmc_bytes_per_walker, &
+          stage_bytes_per_walker
   end type qmc_info
 
   type, bind(C) :: qmc_walkers

@@ -109,6 +109,17 @@ class Context:
         lib().qmc_ctx_info(self.h, C.byref(inf))
         self.info = inf.as_dict()
 
+    def device_bytes(self):
+        """qmc_ctx_info's memory fields now: dev_bytes (what the context
+        holds) and the per-walker bytes of its batch-sized buffers."""
+        inf = abi.QmcInfo()
+        rc = lib().qmc_ctx_info(self.h, C.byref(inf))
+        if rc:
+            raise QmcError(rc, "qmc_ctx_info")
+        d = inf.as_dict()
+        return {k: d[k] for k in ("dev_bytes", "eval_bytes_per_walker", "vmc_bytes_per_walker",
+                                  "stage_bytes_per_walker")}
+
     def set_twist(self, gvec):
         self.gvec = np.ascontiguousarray(gvec, dtype=np.float64).reshape(-1, 2)
         rc = lib().qmc_set_twist(self.h, abi.dptr(self.gvec), self.gvec.shape[0])
@@ -184,6 +195,12 @@ class Context:
 
 class Walkers:
     """Device walker state of a batched VMC run (torch tensors on cuda)."""
+
+    @staticmethod
+    def state_bytes(nw, nptot, n_props, block_sums=True):
+        """Bytes of the tensors __init__ allocates for nw walkers."""
+        per = 8 * (2 * nptot * 2 + 2 + n_props + 1 + 1 + 1) + (8 * n_props if block_sums else 0)
+        return nw * per
 
     def __init__(self, ctx, x, rng_states, block_sums=True):
         import torch

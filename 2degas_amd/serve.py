@@ -6,8 +6,12 @@ local energy per electron reaches the requested target, measured with the
 reference's block error (averages(3), reptation.f:4324-4346:
 err = sqrt(|<b^2> - <b>^2| / (n - 1)) over n block means b).
 
-Jobs are admitted in arrival order while their estimated device footprint
-fits the memory budget, keep their walkers resident on the device, and run
+Jobs are admitted in arrival order while their device memory fits the
+budget: a job's context is created first (its tables go to the device) and
+the library reports, through qmc_ctx_info, what the context holds and what
+each walker adds to its workspace and VMC buffers; with the walker state
+that is exactly what the job allocates.  Admitted jobs keep their walkers
+resident on the device, and run
 concurrently, each on its own HIP stream: a job's next block of VMC sweeps
 is queued as soon as its previous block has finished (polled through an
 event), so small and large systems progress at their own pace while the
@@ -59,22 +63,22 @@ class JobResult:
     done: float = 0.0
     evals: int = 0
     converged: bool = False
+    predicted_bytes: int = 0   # device memory admission counted for the job
+    device_bytes: int = 0      # what its context and walkers held at the end
 
     @property
     def latency(self):
         return self.done - self.arrival
 
 
-def footprint_bytes(spec):
-    """Device memory of a job: per-walker workspace, F (and the LDS path's A)
-    for both spins, walker state, proposal buffers, plus the tables.  An
-    estimate used for admission; the context allocates exactly its layout."""
-    N = spec.nup + spec.ndown
-    ms = [m for m in (spec.nup, spec.ndown) if m > 0]
-    nprops = 16
-    per_w = 8 * (2 * N * 10 + 3 * max(ms + [1]) + nprops * 3 + 64)   # ws + state + proposals
-    per_w += sum(2 * 2 * m * m * 16 + m * m * 16 for m in ms)         # F (+ A)
-    return spec.nw * per_w + (4 << 20)
+def job_bytes(ctx, nw):
+    """Device memory of a VMC job of nw walkers on context ctx: what the
+    context holds (tables, g-vectors), its evaluation workspace and VMC
+    proposal buffers for nw walkers (per-walker sizes from qmc_ctx_info) and
+    the walker state tensors."""
+    m = ctx.device_bytes()
+    return (m["dev_bytes"] + nw * (m["eval_bytes_per_walker"] + m["vmc_bytes_per_walker"])
+            + hip.Walkers.state_bytes(nw, ctx.deck.nptot, ctx.info["n_props"]))
 
 
 def block_error(blocks):
@@ -89,18 +93,23 @@ def block_error(blocks):
 
 
 class _Running:
-    """A job admitted to the device."""
+    """A job: its deck and context from construction (the tables on the
+    device, the memory it will need known), its walkers from start()."""
 
-    def __init__(self, spec, workdir, t_admit):
-        import torch
+    def __init__(self, spec, workdir):
         self.spec = spec
-        self.t_admit = t_admit
         ddir = os.path.join(workdir, f"job{spec.job_id}")
         deckgen.make_deck(ddir, "job", spec.nup, spec.ndown, spec.rs, spec.shells,
                           param_seed=spec.param_seed, perturb=spec.perturb)
         self.deck = deck.read_deck(ddir)
         self.ctx = hip.Context(self.deck, twist=(1, 1))
         self.jetot = self.ctx.info["jetot"] - 1
+        self.need = job_bytes(self.ctx, spec.nw)
+
+    def start(self, t_admit):
+        import torch
+        spec = self.spec
+        self.t_admit = t_admit
         x = deckgen.lattice_configs(spec.nw, self.deck.nptot, self.deck.el[0],
                                     1000 + spec.param_seed)
         s = list(spec.seed_limbs)
@@ -146,9 +155,11 @@ class _Running:
         if np.any(st & 2):
             raise RuntimeError(f"job {self.spec.job_id}: singular orbital matrix")
         err = block_error(self.blocks)
+        held = self.ctx.device_bytes()["dev_bytes"] + hip.Walkers.state_bytes(
+            self.spec.nw, self.deck.nptot, self.ctx.info["n_props"])
         return JobResult(self.spec.job_id, float(np.mean(self.blocks)) if self.blocks else math.nan,
                          err, list(self.blocks), self.spec.arrival, self.t_admit, t_done,
-                         self.evals, err <= self.spec.target)
+                         self.evals, err <= self.spec.target, self.need, held)
 
     def close(self):
         self.stream.synchronize()
@@ -158,26 +169,30 @@ class _Running:
 
 def serve(specs, mem_budget=64 << 30, clock=None, max_active=64, poll_s=0.0005):
     """Run the jobs to their error targets with continuous batching.
-    Returns (results by job id, summary dict)."""
+    Returns (results by job id, summary dict).  The latency percentiles are
+    over the jobs that reached their target; `capped` counts the jobs that
+    stopped at max_blocks without it."""
     import torch
     clock = clock or time.perf_counter
     pending = sorted(specs, key=lambda s: (s.arrival, s.job_id))
     workdir = tempfile.mkdtemp(prefix="degas_serve_")
     active, results = [], {}
     used = 0
+    head = None   # the next job, its context created, waiting for memory
     t0 = clock()
     blocks_run = 0
     while pending or active:
         now = clock() - t0
-        # admission in arrival order while the footprint fits
+        # admission in arrival order while the job's memory fits
         while pending and pending[0].arrival <= now and len(active) < max_active:
-            fp = footprint_bytes(pending[0])
-            if active and used + fp > mem_budget:
+            if head is None:
+                head = _Running(pending[0], workdir)
+            if active and used + head.need > mem_budget:
                 break
-            spec = pending.pop(0)
-            r = _Running(spec, workdir, clock() - t0)
-            r.fp = fp
-            used += fp
+            pending.pop(0)
+            r, head = head, None
+            r.start(clock() - t0)
+            used += r.need
             r.queue_block()
             active.append(r)
         if not active:
@@ -194,7 +209,7 @@ def serve(specs, mem_budget=64 << 30, clock=None, max_active=64, poll_s=0.0005):
             if r.finish_block():
                 res = r.result(clock() - t0)
                 results[res.job_id] = res
-                used -= r.fp
+                used -= r.need
                 r.close()
             else:
                 r.queue_block()
@@ -204,10 +219,11 @@ def serve(specs, mem_budget=64 << 30, clock=None, max_active=64, poll_s=0.0005):
             time.sleep(poll_s)
     torch.cuda.synchronize()
     wall = clock() - t0
-    lat = np.array([r.latency for r in results.values()])
+    lat = np.array([r.latency for r in results.values() if r.converged])
     evals = sum(r.evals for r in results.values())
     summary = {
         "jobs": len(results),
+        "capped": int(sum(not r.converged for r in results.values())),
         "wall_s": wall,
         "jobs_per_s": len(results) / wall if wall > 0 else 0.0,
         "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,

@@ -145,6 +145,8 @@ def main():
                          "captured hipGraph and launched directly (QMC_NO_GRAPH)")
     ap.add_argument("--jobs", type=int, default=40)
     ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--max-blocks", type=int, default=1000,
+                    help="--serve: blocks after which a job stops without its target")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 1 GiB device-to-device copies: the known-size "
                          "kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
@@ -356,7 +358,7 @@ def bench_serve(args):
     to the requested error bar, jobs/s and aggregate local-energy evals/s."""
     sv = importlib.import_module("2degas_amd.serve")
     specs = sv.poisson_jobs(args.jobs, args.rate, seed=2024, nw=256, target=2e-4,
-                            max_blocks=200, steps_per_block=20)
+                            max_blocks=args.max_blocks, steps_per_block=20)
     results, summary = sv.serve(specs)
     out = {
         "metric": "served VMC jobs: p50/p99 time to 1e-4 Ha/electron",
@@ -375,7 +377,14 @@ def bench_serve(args):
         "jobs_per_s": summary["jobs_per_s"],
         "evals_per_s": summary["evals_per_s"],
         "converged": summary["converged"],
+        "capped": summary["capped"],
+        "latency_over": "jobs that reached the target (capped jobs stopped at max_blocks)",
         "wall_s": summary["wall_s"],
+        "max_blocks": args.max_blocks,
+        "per_job": [{"job": r.job_id, "N": s.nup + s.ndown, "nup": s.nup, "rs": round(s.rs, 3),
+                     "blocks": len(r.blocks), "error": r.error, "converged": r.converged,
+                     "latency_s": r.latency, "device_bytes": r.device_bytes}
+                    for s in specs for r in [results[s.job_id]]],
     }
     print(json.dumps(out))
 

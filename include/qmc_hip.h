@@ -88,6 +88,18 @@ typedef struct qmc_info {
   int32_t jpot[2], jkin[2], jun[2];
   int32_t ng;              /* g-vectors currently set                       */
   int32_t has_det;         /* backflow determinant active                  */
+  int32_t reserved0;       /* zero (aligns the byte counts below)          */
+  /* Device memory, for callers that budget it (admission of jobs): what
+   * the context holds now (tables, g-vectors and every per-walker buffer
+   * allocated so far, each kept at its largest size), and what each walker
+   * adds to the buffers that grow with the batch: the evaluation workspace
+   * (every entry point), the VMC proposal buffers (qmc_vmc_*, qmc_rmc_*) and
+   * the staging of the host-pointer qmc_eval_batch.  The walker state of
+   * qmc_walkers is the caller's and is not counted.                        */
+  uint64_t dev_bytes;
+  uint64_t eval_bytes_per_walker;
+  uint64_t vmc_bytes_per_walker;
+  uint64_t stage_bytes_per_walker;
 } qmc_info;
 
 typedef struct qmc_ctx qmc_ctx;

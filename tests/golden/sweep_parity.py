@@ -58,17 +58,21 @@ def _orc_chunk(args):
 def _vmc_chain(args):
     name, k, nmoves, delta = args
     d = deck.read_deck(os.path.join(DECKS, name))
-    x0 = deckgen.lattice_configs(1, d.nptot, d.el[0], 500 + k)[0]
-    seed = list(d.seed)
-    seed[3] = (seed[3] + 17 * k) % 4096
-    raw, _ = po.run_refeval(os.path.join(DECKS, name), os.path.join(WORK, f"{name}_v{k}"), 2,
-                            x0[None], delta=delta, nsteps=nmoves, seed=seed)
-    r = po.parse_replay(raw, d.nptot, nmoves)
-    acc = float(np.mean(r["age"][200:] == 0))
-    if acc < 0.2:
-        raise RuntimeError(f"VMC chain {k} of {name} accepts {acc:.3f} of its moves at delta "
-                           f"{delta}: lower the time step")
-    return r["x_old"][200::10]
+    # a start next to a node of the trial function can leave a single
+    # walker stuck (every Langevin move rejected): such a chain is started
+    # again from the next jittered grid, at most four times
+    for attempt in range(4):
+        x0 = deckgen.lattice_configs(1, d.nptot, d.el[0], 500 + k + 1000 * attempt)[0]
+        seed = list(d.seed)
+        seed[3] = (seed[3] + 17 * k + 5 * attempt) % 4096
+        raw, _ = po.run_refeval(os.path.join(DECKS, name), os.path.join(WORK, f"{name}_v{k}"), 2,
+                                x0[None], delta=delta, nsteps=nmoves, seed=seed)
+        r = po.parse_replay(raw, d.nptot, nmoves)
+        acc = float(np.mean(r["age"][200:] == 0))
+        if acc >= 0.2:
+            return r["x_old"][200::10]
+    raise RuntimeError(f"VMC chain {k} of {name} accepts {acc:.3f} of its moves at delta "
+                       f"{delta} from four starts: lower the time step")
 
 
 def _merge(parts):

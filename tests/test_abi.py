@@ -108,3 +108,55 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
         assert int(got[cname]) == ctypes.sizeof(py), cname
         for fname, _ in py._fields_:
             assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, f"{cname}.{fname}"
+
+
+def _fortran_compiler():
+    import shutil
+    for fc in ("gfortran", "amdflang", "/opt/rocm/bin/amdflang"):
+        if shutil.which(fc) or os.path.exists(fc):
+            return shutil.which(fc) or fc
+    return None
+
+
+def test_fortran_bind_c_types_match_the_c_layout(tmp_path):
+    """Sizes (c_sizeof) and component offsets (c_loc differences) of the
+    bind(C) types of fortran/qmc_hip_mod.f90, compiled with the Fortran
+    compiler of the image, equal the ctypes mirrors (which equal the C
+    header's layout, checked above)."""
+    import ctypes
+    import subprocess
+    fc = _fortran_compiler()
+    if fc is None:
+        pytest.skip("no Fortran compiler")
+    abi = pkg("abi")
+    structs = {"qmc_tables": abi.QmcTables, "qmc_info": abi.QmcInfo,
+               "qmc_walkers": abi.QmcWalkers, "qmc_reptiles": abi.QmcReptiles,
+               "qmc_rmc_params": abi.QmcRmcParams}
+    lines = ["program probe", "  use iso_c_binding", "  use qmc_hip_mod", "  implicit none"]
+    for cname in structs:
+        lines.append(f"  type({cname}), target :: v_{cname}")
+    lines.append("  integer(c_intptr_t) :: base")
+    for cname, py in structs.items():
+        v = f"v_{cname}"
+        lines.append(f"  print '(a,1x,i0)', '{cname}', c_sizeof({v})")
+        lines.append(f"  base = transfer(c_loc({v}), base)")
+        for fname, _ in py._fields_:
+            lines.append(f"  print '(a,1x,i0)', '{cname}.{fname}', "
+                         f"transfer(c_loc({v}%{fname}), base) - base")
+    lines.append("end program probe")
+    src = tmp_path / "probe.f90"
+    src.write_text("\n".join(lines) + "\n")
+    mod = os.path.join(ROOT, "2degas_amd", "fortran", "qmc_hip_mod.f90")
+    subprocess.check_call([fc, "-c", mod, "-o", str(tmp_path / "qmc_hip_mod.o")], cwd=tmp_path)
+    exe = tmp_path / "probe"
+    lib = os.path.join(ROOT, "2degas_amd", "lib2degas_hip.so")
+    if not os.path.exists(lib):
+        pytest.skip("lib2degas_hip.so not built")
+    # the module's error-text helper calls qmc_last_error: link the library
+    subprocess.check_call([fc, str(src), str(tmp_path / "qmc_hip_mod.o"), lib,
+                           "-Wl,-rpath," + os.path.dirname(lib), "-o", str(exe)], cwd=tmp_path)
+    got = dict(l.split() for l in subprocess.check_output([str(exe)]).decode().splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, f"{cname}.{fname}"

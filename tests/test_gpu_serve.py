@@ -31,4 +31,32 @@ def test_jobs_give_the_same_results_alone_and_in_the_mix():
         assert len(blocks) == 5
         assert np.all(np.isfinite(blocks))
         assert mixed[j].blocks == blocks, f"job {j} differs in the mix"
+    # every job stopped at max_blocks: none reached its target, so there are
+    # no time-to-target latencies
+    assert summary["capped"] == 3 and summary["converged"] == 0
+    assert summary["p50_latency_s"] is None
+
+
+def test_admission_counts_what_each_job_allocates():
+    """Each job's admission bytes (qmc_ctx_info's sizes for its walker count
+    plus its walker tensors) equal what its context and walkers held when it
+    retired.  With a budget below any one job, jobs run one at a time in
+    arrival order; with a reachable target every job converges and the
+    latency percentiles are over all of them."""
+    print("\n[gpu] serve admission by allocation", flush=True)
+    sv = pkg("serve")
+    specs = _specs(sv)
+    for s in specs:
+        s.target, s.min_blocks, s.max_blocks = 10.0, 2, 5
+    res, summary = sv.serve(specs, mem_budget=1)
+    print(f"   summary {summary}", flush=True)
+    order = sorted(res.values(), key=lambda r: r.admitted)
+    for r in order:
+        print(f"   job {r.job_id}: predicted {r.predicted_bytes} held {r.device_bytes}", flush=True)
+        assert r.predicted_bytes == r.device_bytes > 0
+        assert r.converged and len(r.blocks) == 2
+    assert [r.job_id for r in order] == [0, 1, 2]
+    for a, b in zip(order, order[1:]):
+        assert b.admitted >= a.done
+    assert summary["capped"] == 0 and summary["converged"] == 3
     assert summary["p99_latency_s"] >= summary["p50_latency_s"] > 0

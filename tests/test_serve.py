@@ -1,5 +1,6 @@
 """Host logic of the job service (2degas_amd/serve.py): the reference's
-block error, admission footprints and the arrival generator."""
+block error, the walker-state bytes admission counts and the arrival
+generator."""
 import math
 
 import numpy as np
@@ -21,14 +22,15 @@ def test_block_error_matches_reference_formula():
     assert math.isinf(sv.block_error([1.0]))
 
 
-def test_footprint_grows_with_walkers_and_size():
-    sv = pkg("serve")
-    a = sv.footprint_bytes(sv.JobSpec(0, 13, 13, 5.0, nw=128))
-    b = sv.footprint_bytes(sv.JobSpec(0, 13, 13, 5.0, nw=256))
-    c = sv.footprint_bytes(sv.JobSpec(0, 61, 61, 5.0, nw=256))
-    assert a < b < c
-    # F for 61+61 at 256 walkers alone is 2 spins x 2 x 61^2 x 16 B x 256
-    assert c > 2 * 2 * 61 * 61 * 16 * 256
+def test_walker_state_bytes_are_the_tensors_walkers_allocates():
+    """hip.Walkers.state_bytes: x and g (nw x N x 2 doubles each), h (nw x 2),
+    props and block sums (nw x n_props each), grad2_ph, age and the RNG
+    state (one 8-byte word each per walker)."""
+    hip = pkg("hip")
+    nw, N, npr = 256, 122, 40
+    want = nw * 8 * (2 * N * 2 + 2 + npr + 1 + 1 + 1 + npr)
+    assert hip.Walkers.state_bytes(nw, N, npr) == want
+    assert hip.Walkers.state_bytes(nw, N, npr, block_sums=False) == want - nw * 8 * npr
 
 
 def test_poisson_arrivals_and_job_mix():
