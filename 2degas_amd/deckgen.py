@@ -109,14 +109,20 @@ def write_v_table(path, rows, drt, vk):
 
 def make_deck(dirpath, runid, nup, ndown, rs, nshell_ewald, nshell_k=None, ngrid=200,
               ngrid_u3=100, u3=True, backflow="yk", eta_amp=0.2, ntwists=0,
-              seed=(3535, 3467, 3262, 1549), param_seed=1, perturb=0.0):
+              seed=(3535, 3467, 3262, 1549), param_seed=1, perturb=0.0, jastrow="n26"):
     """Write a complete deck with the reference's own generators
     (2degas_amd/egas.py); returns a dict of its physical parameters.
 
     * box and units as egas's main program (setup.f:20-44): density 1/pi in
       units of rs, L = (N pi)^(1/2), hbs2m = 1/rs^2, drt = L/(2 ngrid);
     * u2: u2_ob_cos with the cusp -rs, as egas's main program builds it
-      (setup.f:108-112), and the other parameters of the shipped n26.u2;
+      (setup.f:108-112), and the other parameters of the shipped n26.u2
+      (jastrow="n26"; optimized for N=26 at rs=5, where the function
+      decreases over the whole table; at larger boxes its rational part
+      turns up again before L/2, a well that draws electrons together), or
+      jastrow="pade": u2_ob_cos with only the cusp and the range,
+      u(r) = -rs r / (1 + r) + const (b = d = 0, c = 1, no cosine terms),
+      which decreases to zero at L/2 for every N and rs;
     * u3 (optional): u2_reatto with the parameters of n26.u3 on ngrid_u3
       points of the same drt (one drt serves every table, reptation.f:4705);
     * backflow: "yk" is eta_yk with the parameters of n26.b, "gauss" is
@@ -145,7 +151,12 @@ def make_deck(dirpath, runid, nup, ndown, rs, nshell_ewald, nshell_k=None, ngrid
         raise ValueError("Ewald shells and k list disagree")
     vrows = egas.spline(ew["vr"], ew["dvr"], drt, 1)
     write_v_table(os.path.join(dirpath, runid + ".v"), vrows, drt, ew["vk"])
-    u2p = [N26_U2[0] * rs] + [pert(v) for v in N26_U2[1:]]
+    if jastrow == "n26":
+        u2p = [N26_U2[0] * rs] + [pert(v) for v in N26_U2[1:]]
+    elif jastrow == "pade":
+        u2p = [-rs, 0.0, pert(1.0), 0.0, 1.0, 0.0, 0.0, 0.0]
+    else:
+        raise ValueError(f"unknown jastrow {jastrow!r}")
     egas.write_tgen_table(os.path.join(dirpath, runid + ".u2"),
                           egas.tgen("u2_ob_cos", u2p, ngrid, drt), drt, "u2_ob_cos", u2p,
                           N26_U2_FLAGS)

@@ -1,3 +1,35 @@
+      module qmc_hip_host
+c The device context the Fortran forwarders share (created by the first
+c compute_properties call from the deck state in COMMON) and the HIP
+c runtime calls they use for device buffers.
+      use iso_c_binding
+      use qmc_hip_mod
+      implicit none
+      type(c_ptr), save :: ctx = c_null_ptr
+      type(qmc_info), save :: info
+      integer(c_int), parameter :: hipMemcpyHostToDevice = 1
+      integer(c_int), parameter :: hipMemcpyDeviceToHost = 2
+      interface
+       integer(c_int) function hipMalloc(p, bytes)
+     &   bind(C, name='hipMalloc')
+        import :: c_int, c_ptr, c_size_t
+        type(c_ptr) :: p
+        integer(c_size_t), value :: bytes
+       end function hipMalloc
+       integer(c_int) function hipFree(p) bind(C, name='hipFree')
+        import :: c_int, c_ptr
+        type(c_ptr), value :: p
+       end function hipFree
+       integer(c_int) function hipMemcpy(dst, src, bytes, kind)
+     &   bind(C, name='hipMemcpy')
+        import :: c_int, c_ptr, c_size_t
+        type(c_ptr), value :: dst, src
+        integer(c_size_t), value :: bytes
+        integer(c_int), value :: kind
+       end function hipMemcpy
+      end interface
+      end module qmc_hip_host
+
       subroutine compute_properties(ifdist)
 c Drop-in replacement of the reference's compute_properties
 c (reptation.f:2661-2727).  Same name, same argument, same COMMON inputs
@@ -12,12 +44,11 @@ c again whenever tabc/twistg change it.  The pair arrays of /c_dist/ are
 c not filled (only gofr and ltf_bf2 read them, both outside this path).
       use iso_c_binding
       use qmc_hip_mod
+      use qmc_hip_host
       implicit none
       include 'ewald.h'
       integer ifdist
-      type(c_ptr), save :: ctx = c_null_ptr
       type(qmc_tables), save :: tab
-      type(qmc_info), save :: info
       real(c_double), allocatable, target, save :: ut_c(:,:,:)
       integer(c_int32_t), allocatable, target, save :: ngrid_c(:)
       integer(c_int32_t), allocatable, target, save :: iexp_c(:)

@@ -48,6 +48,17 @@ class JobSpec:
     min_blocks: int = 4
     max_blocks: int = 400
     shells: int = 20           # Ewald k-shells of the generated deck
+    # the generated deck (deckgen.make_deck): a two-body Jastrow that
+    # decreases to zero at L/2 for every N and rs (the n26 parameters turn
+    # into an attractive well at larger boxes), no three-body term, the n26
+    # backflow; the free parameters perturbed by `perturb`
+    jastrow: str = "pade"
+    u3: bool = False
+    backflow: str = "yk"
+    # consecutive rejections after which a move is accepted (the reference's
+    # mage, metropolis_test): walkers started next to a node, where every
+    # proposal is rejected, move again instead of freezing their energy
+    mage: float = 50.0
     arrival: float = 0.0       # seconds after the service starts
     seed_limbs: tuple = (3535, 3467, 3262, 1549)
 
@@ -100,6 +111,7 @@ class _Running:
         self.spec = spec
         ddir = os.path.join(workdir, f"job{spec.job_id}")
         deckgen.make_deck(ddir, "job", spec.nup, spec.ndown, spec.rs, spec.shells,
+                          jastrow=spec.jastrow, u3=spec.u3, backflow=spec.backflow,
                           param_seed=spec.param_seed, perturb=spec.perturb)
         self.deck = deck.read_deck(ddir)
         self.ctx = hip.Context(self.deck, twist=(1, 1))
@@ -130,7 +142,8 @@ class _Running:
         import torch
         with torch.cuda.stream(self.stream):
             self.walkers.blk.zero_()
-            self.walkers.sweep(self.spec.steps_per_block, self.delta, stream=self.stream)
+            self.walkers.sweep(self.spec.steps_per_block, self.delta, mage=self.spec.mage,
+                               stream=self.stream)
             self.event.record(self.stream)
 
     def finish_block(self):

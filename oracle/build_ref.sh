@@ -70,6 +70,17 @@ if [ -f "$PKG/lib2degas_hip.so" ]; then
       -L"$PKG" -l2degas_hip -Wl,-rpath,'$ORIGIN/../../2degas_amd' -Wl,-rpath,/opt/rocm/lib \
       -o "$OUT/refqmc_hip"
   echo "build_ref: built $OUT/refqmc_hip (reference driver + GPU compute_properties)"
+  # and with the reference's vmc replaced by the device VMC driver
+  # (2degas_amd/fortran/vmc_hip.f, the Fortran caller of qmc_vmc_sweep)
+  sed -e 's/^      subroutine vmc$/      subroutine vmc_cpu/' reptation_dropin.f > reptation_dropvmc.f
+  grep -q '^      subroutine vmc_cpu$' reptation_dropvmc.f || { echo "build_ref: vmc rename failed" >&2; exit 1; }
+  $FC $FFLAGS -c reptation_dropvmc.f -o reptation_dropvmc.o
+  cp "$PKG/fortran/vmc_hip.f" .
+  $FC $FFLAGS -c vmc_hip.f -o vmc_hip.o
+  $FC $FFLAGS refmain.o reptation_dropvmc.o mpif.o compute_properties_hip.o vmc_hip.o qmc_hip_mod.o \
+      -L"$PKG" -l2degas_hip -L/opt/rocm/lib -lamdhip64 -Wl,-rpath,'$ORIGIN/../../2degas_amd' \
+      -Wl,-rpath,/opt/rocm/lib -o "$OUT/refqmc_hipvmc"
+  echo "build_ref: built $OUT/refqmc_hipvmc (reference driver + GPU compute_properties + GPU vmc)"
 fi
 echo "$FC $($FC --version 2>/dev/null | head -1) $OPT" > "$OUT/COMPILER"
 echo "build_ref: built $OUT/refeval $OUT/refqmc with $FC $OPT"

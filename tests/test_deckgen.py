@@ -84,3 +84,27 @@ def test_breakup_energy_matches_ewald_sum():
     eew = (e2 * erfc(alpha * r) / r).sum() + (2 * np.pi * e2 / (A * k) * erfc(k / (2 * alpha)) * rho2).sum() \
         - N * alpha * e2 / np.sqrt(np.pi) - N * N * e2 * np.sqrt(np.pi) / (alpha * A)
     assert abs(ebreak - eew) < 2e-5 * N * e2, (ebreak, eew)
+
+
+def test_pade_jastrow_decreases_to_zero_for_every_size(tmp_path):
+    """jastrow="pade" (the served jobs' u2): the table decreases over the
+    whole range and vanishes at L/2 for N from 26 to 122 and rs from 1 to
+    20.  The n26 parameters do not at N=122, rs=5: their rational part
+    turns up before L/2 (a well between electrons)."""
+    import numpy as np
+    dg = pkg("deckgen")
+    deck = pkg("deck")
+    for n, (nu, nd) in enumerate(((13, 13), (37, 0), (61, 61))):
+        for rs in (1.0, 5.0, 20.0):
+            dd = str(tmp_path / f"p{n}_{rs}")
+            dg.make_deck(dd, "job", nu, nd, rs, 5, u3=False, backflow="zero", jastrow="pade",
+                         param_seed=3, perturb=0.1)
+            d = deck.read_deck(dd)
+            f = d.tables[d.iu2[0] - 1].coef[:, 0]
+            assert np.all(np.diff(f) <= 1e-12), (nu, nd, rs)
+            assert abs(f[-1]) < 1e-12 and f[0] > 0
+    dd = str(tmp_path / "n26_122")
+    dg.make_deck(dd, "job", 61, 61, 5.0, 5, u3=False, backflow="zero")
+    d = deck.read_deck(dd)
+    f = d.tables[d.iu2[0] - 1].coef[:, 0]
+    assert np.any(np.diff(f) > 1e-6) and f.min() < -0.5
