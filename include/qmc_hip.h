@@ -164,9 +164,12 @@ int qmc_eval_batch(qmc_ctx* ctx, int nw, const double* x, double* props,
 /*
  * Same as qmc_eval_batch with all arrays already in device memory and the
  * work queued on `stream` (a hipStream_t, NULL = default stream); returns
- * once queued.  `precision` must be QMC_PREC_FP64; QMC_PREC_MIXED is reserved
- * and returns QMC_ERR_ARG (no FP32/FP64 split of this path keeps the local
- * energy within 1e-6 of FP64 at a useful speedup).
+ * once queued.  `precision` is QMC_PREC_FP64, or QMC_PREC_MIXED: the u2, u3
+ * and pair-potential tables evaluated in FP32 and the rho_k phases formed
+ * and summed in FP32 over chunks of particles, each chunk's sum added into
+ * FP64; the geometry, backflow, determinant and every accumulator stay FP64
+ * (local energy within about 1e-6 relative of FP64 on sampled
+ * configurations).
  */
 int qmc_eval_batch_device(qmc_ctx* ctx, int nw, const double* d_x,
                           double* d_props, double* d_g, double* d_h,

@@ -6,12 +6,13 @@ double (the reference's loop order, its own FP64 rounding) and the oracle
 in long double (the exact value of the same formulas for the same double
 inputs).  Errors are per configuration (conftest.config_errors, h_errors).
 
-The reference's own FP64 arithmetic misses the exact value by up to ~2e-9
-on VMC-sampled N=122 configurations and up to 3e-8 on uniform ones (the
-committed tests/golden/sweep_<deck>.json, made by tests/golden/sweep_parity.py
-against the compiled reference; the double oracle follows its loop order
-and its errors).  A 1e-10 bar against the reference is therefore not
-well-posed; the HIP path is held to the exact value instead:
+The reference's own FP64 arithmetic misses the exact value by up to 1e-9
+on VMC-sampled N=122 configurations (2e-13 at N=26 and 58) and up to 3e-8
+on uniform ones (the committed tests/golden/sweep_<deck>.json, made by
+tests/golden/sweep_parity.py against the compiled reference; the double
+oracle follows its loop order and its errors).  A 1e-10 bar against the
+reference is therefore not well-posed at every shape; the HIP path is held
+to the exact value instead:
 
 * on every configuration, |HIP - exact| <= 1e-10 + 2 |oracle - exact|
   (within 1e-10 of the exact value, or of the order of the reference's

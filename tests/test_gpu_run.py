@@ -329,3 +329,40 @@ def test_rmc_reptiles_are_ranks(tmp_path):
     for b in (1, 2):
         _close([v[0] for v, _ in bo[b]], np.mean(np.array(means[b]), axis=0), 1e-8,
                f"rmc block {b} means")
+
+
+REF_HIP = os.path.join(os.path.dirname(REF), "refqmc_hip")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_HIP),
+                    reason="drop-in driver not built (needs the reference sources)")
+def test_reference_rmc_driver_with_gpu_compute_properties(tmp_path):
+    """The drop-in on a reptation run: the reference's own rmc driver
+    (reptation.f:1286-1313) linked with the GPU compute_properties against
+    the same driver on the CPU, one rank, two blocks: block lines, `.res`
+    sums and limbs, and the final path agree."""
+    print("\n[gpu] drop-in reference rmc driver", flush=True)
+    x0 = pkg("deck").read_configs([os.path.join(DECKS, "n26g", f"n26.{t}.x.res.0") for t in TYPES],
+                                  [13, 13])[0]
+    paths = _paths(1, x0, 79)
+    cpu, gpu = str(tmp_path / "cpu"), str(tmp_path / "gpu")
+    outs = {}
+    for w, exe in ((cpu, REF), (gpu, REF_HIP)):
+        _rmc_deck(w, paths=paths)
+        _script(w, RMC_REC.format(nblk=2) + "end\n")
+        r = subprocess.run([exe], cwd=w, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=300)
+        assert r.returncode == 0, r.stdout.decode()[-2000:]
+        outs[w] = r.stdout.decode()
+    bc, bg = _blocks_rmc(outs[cpu]), _blocks_rmc(outs[gpu])
+    assert sorted(bc) == sorted(bg) == [1, 2]
+    for b in bc:
+        _close_lines(bg[b], bc[b], 1e-8, f"rmc block {b}")
+    tc, tg = _res(cpu), _res(gpu)
+    assert tc[:2] == tg[:2] == ["2", "rmc"] and len(tc) == len(tg)
+    _close([float(v) for v in tg[2:-8]], [float(v) for v in tc[2:-8]], 1e-8, "restart sums")
+    assert tc[-8:] == tg[-8:]
+    for t in TYPES:
+        a = np.loadtxt(os.path.join(gpu, f"n26.{t}.x.0"))
+        b = np.loadtxt(os.path.join(cpu, f"n26.{t}.x.0"))
+        assert a.shape == b.shape
+        assert np.max(np.abs(a - b)) < 1e-8
