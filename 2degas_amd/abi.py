@@ -151,6 +151,28 @@ class QmcRmcParams(C.Structure):
     ]
 
 
+class QmcDmcParams(C.Structure):
+    _fields_ = [
+        ("delta", C.c_double),
+        ("etrial", C.c_double),
+        ("mage", C.c_double),
+        ("mmstep", C.c_int32),
+        ("mmult", C.c_int32),
+        ("bm_order", C.c_int32),
+        ("reserved0", C.c_int32),
+    ]
+
+
+class QmcDmcState(C.Structure):
+    _fields_ = [
+        ("wate", C.c_void_p),
+        ("wsum", C.c_void_p),
+        ("src", C.c_void_p),
+        ("rng2", C.c_void_p),
+        ("counts", C.c_void_p),
+    ]
+
+
 def limbs_to_state(limbs):
     """48-bit LCG state from the four 12-bit limbs savern writes."""
     l1, l2, l3, l4 = (int(v) for v in limbs)

@@ -73,6 +73,8 @@ struct qmc_ctx {
   int vmc_nw = 0;
   double *d_xn = nullptr, *d_gn = nullptr, *d_hn = nullptr, *d_pn = nullptr, *d_wold = nullptr;
   int32_t *d_stn = nullptr, *d_sticky = nullptr;
+  // DMC: each walker's energy and age before its latest move
+  double *d_eprev = nullptr, *d_aprev = nullptr;
   // the VMC step as a captured graph, and what it was captured with
   bool use_graph = true;
   hipGraphExec_t vmc_exec = nullptr;
@@ -97,6 +99,10 @@ void free_vmc_graph(qmc_ctx* c);
 // Workspace of the evaluation kernels for nw walkers (allocates when it grows).
 int ensure_ws(qmc_ctx* c, int nw);
 int ensure_vmc_scratch(qmc_ctx* c, int nw);
+// One VMC step for every walker of w (proposal, evaluation, acceptance;
+// block sums when w->blk is set), queued on st.
+int vmc_step(qmc_ctx* c, const qmc_walkers* w, hipStream_t st, double delta, double var0, double var1,
+             double mage, int bm_order, double* tp, double* ta);
 void free_rmc(qmc_ctx* c);
 
 

@@ -173,8 +173,11 @@ void free_vmc(qmc_ctx* c) {
   (void)hipFree(c->d_wold);
   (void)hipFree(c->d_stn);
   (void)hipFree(c->d_sticky);
+  (void)hipFree(c->d_eprev);
+  (void)hipFree(c->d_aprev);
   c->d_xn = c->d_gn = c->d_hn = c->d_pn = c->d_wold = nullptr;
   c->d_stn = c->d_sticky = nullptr;
+  c->d_eprev = c->d_aprev = nullptr;
   c->vmc_nw = 0;
 }
 
@@ -190,6 +193,8 @@ int ensure_vmc_scratch(qmc_ctx* c, int nw) {
   HIPCHK(hipMalloc(&c->d_wold, sizeof(double) * nw));
   HIPCHK(hipMalloc(&c->d_stn, sizeof(int32_t) * nw));
   HIPCHK(hipMalloc(&c->d_sticky, sizeof(int32_t) * nw));
+  HIPCHK(hipMalloc(&c->d_eprev, sizeof(double) * nw));
+  HIPCHK(hipMalloc(&c->d_aprev, sizeof(double) * nw));
   HIPCHK(hipMemset(c->d_pn, 0, sizeof(double) * NP * nw));
   // a sweep may run without a start call (state loaded by the caller)
   HIPCHK(hipMemset(c->d_sticky, 0, sizeof(int32_t) * nw));
@@ -221,8 +226,8 @@ extern "C" int qmc_vmc_start(qmc_ctx* c, const qmc_walkers* w, void* stream) {
 }
 
 // One VMC step for every walker: proposal, evaluation, acceptance.
-static int vmc_step(qmc_ctx* c, const qmc_walkers* w, hipStream_t st, double delta, double var0,
-                    double var1, double mage, int bm_order, double* tp, double* ta) {
+int vmc_step(qmc_ctx* c, const qmc_walkers* w, hipStream_t st, double delta, double var0, double var1,
+             double mage, int bm_order, double* tp, double* ta) {
   const DevSys& s = c->s;
   const int nw = w->nw;
   const int wpb = QMC_BLOCK / 64;

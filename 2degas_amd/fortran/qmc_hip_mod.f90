@@ -46,6 +46,15 @@ module qmc_hip_mod
      type(c_ptr) :: x, g, props, age, w, ends, rage, pp_old, grad2_ph, rng, blk
   end type qmc_reptiles
 
+  type, bind(C) :: qmc_dmc_params
+     real(c_double) :: delta, etrial, mage
+     integer(c_int32_t) :: mmstep, mmult, bm_order, reserved0
+  end type qmc_dmc_params
+
+  type, bind(C) :: qmc_dmc_state
+     type(c_ptr) :: wate, wsum, src, rng2, counts
+  end type qmc_dmc_state
+
   type, bind(C) :: qmc_rmc_params
      real(c_double) :: delta, etrial, mage, mage_r
      integer(c_int32_t) :: ntauskip, mdelta, rejection, wpiu, gstorto, nodalaction, ecut, bm_order
@@ -137,6 +146,22 @@ module qmc_hip_mod
        integer(c_int), value :: nw
        integer(c_int32_t), intent(out) :: sticky(*)
      end function qmc_vmc_status
+
+     integer(c_int) function qmc_dmc_sweep(ctx, w, d, nsteps, prm, stream) bind(C, name='qmc_dmc_sweep')
+       import :: c_int, c_ptr, qmc_walkers, qmc_dmc_state, qmc_dmc_params
+       type(c_ptr), value :: ctx, stream
+       type(qmc_walkers), intent(in) :: w
+       type(qmc_dmc_state), intent(in) :: d
+       integer(c_int), value :: nsteps
+       type(qmc_dmc_params), intent(in) :: prm
+     end function qmc_dmc_sweep
+
+     integer(c_int) function qmc_dmc_branch(ctx, nw, wate, rng2, src, counts, stream) &
+          bind(C, name='qmc_dmc_branch')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, wate, rng2, src, counts, stream
+       integer(c_int), value :: nw
+     end function qmc_dmc_branch
 
      integer(c_int) function qmc_rmc_start(ctx, r, prm, stream) bind(C, name='qmc_rmc_start')
        import :: c_int, c_ptr, qmc_reptiles, qmc_rmc_params

@@ -18,7 +18,8 @@ EXPORTS = ("qmc_ctx_create", "qmc_ctx_destroy", "qmc_ctx_info", "qmc_set_twist",
            "qmc_twist_gvec", "qmc_eval_batch", "qmc_eval_batch_device", "qmc_strerror",
            "qmc_last_error", "qmc_flops_per_eval", "qmc_timing", "qmc_timing_read",
            "qmc_vmc_start", "qmc_vmc_sweep", "qmc_vmc_status", "qmc_rmc_start",
-           "qmc_rmc_sweep", "qmc_set_delta", "qmc_eval_batch_device_noalloc")
+           "qmc_rmc_sweep", "qmc_set_delta", "qmc_eval_batch_device_noalloc", "qmc_dmc_sweep",
+           "qmc_dmc_branch")
 
 _lib = None
 
@@ -71,6 +72,9 @@ def lib():
         L.qmc_rmc_sweep.argtypes = [vp, C.POINTER(abi.QmcReptiles), C.POINTER(abi.QmcRmcParams),
                                     C.c_int, vp, vp]
         L.qmc_timing_read.argtypes = [vp, dp, C.POINTER(C.c_int)]
+        L.qmc_dmc_sweep.argtypes = [vp, C.POINTER(abi.QmcWalkers), C.POINTER(abi.QmcDmcState), C.c_int,
+                                    C.POINTER(abi.QmcDmcParams), vp]
+        L.qmc_dmc_branch.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -260,6 +264,49 @@ class Walkers:
         if rc:
             raise QmcError(rc, "qmc_vmc_status")
         return st
+
+
+class Dmc:
+    """Device state of a DMC run over a Walkers set (qmc_dmc_state): the
+    step weights, their block sums, the last branching's sources, the second
+    generator (seeded from the 4 limbs setrn2 keeps) and the counters."""
+
+    def __init__(self, walkers, limbs2):
+        import torch
+        self.wk = walkers
+        nw, dev = walkers.nw, walkers.x.device
+        self.wate = torch.ones(nw, dtype=torch.float64, device=dev)
+        self.wsum = torch.zeros(nw, dtype=torch.float64, device=dev)
+        self.src = torch.zeros(nw, dtype=torch.int32, device=dev)
+        self.rng2 = torch.tensor([abi.limbs_to_state(limbs2)], dtype=torch.int64, device=dev)
+        self.counts = torch.zeros(4, dtype=torch.int32, device=dev)
+        s = abi.QmcDmcState()
+        for name in ("wate", "wsum", "src", "rng2", "counts"):
+            setattr(s, name, getattr(self, name).data_ptr())
+        self.s = s
+
+    def sweep(self, nsteps, delta, etrial, mmstep=1, mmult=10, mage=1e30, bm_order=0, stream=None):
+        """nsteps DMC steps (qmc_dmc_sweep); the walkers' block sums get
+        p_old times the weight, wsum the weights."""
+        p = abi.QmcDmcParams()
+        p.delta, p.etrial, p.mage = float(delta), float(etrial), float(mage)
+        p.mmstep, p.mmult, p.bm_order, p.reserved0 = int(mmstep), int(mmult), int(bm_order), 0
+        rc = lib().qmc_dmc_sweep(self.wk.ctx.h, C.byref(self.wk.s), C.byref(self.s), nsteps, C.byref(p),
+                                 self.wk._stream(stream))
+        if rc:
+            raise QmcError(rc, "qmc_dmc_sweep")
+
+
+def dmc_branch(ctx, wate, rng2, src, counts, stream=None):
+    """qmc_dmc_branch on device tensors (wate float64 [nw], overwritten;
+    rng2 int64 [1]; src int32 [nw]; counts int32 [4])."""
+    import torch
+    st = stream if stream is not None else torch.cuda.current_stream()
+    rc = lib().qmc_dmc_branch(ctx.h, int(wate.numel()), C.c_void_p(wate.data_ptr()), C.c_void_p(rng2.data_ptr()),
+                              C.c_void_p(src.data_ptr()), C.c_void_p(counts.data_ptr()),
+                              C.c_void_p(st.cuda_stream))
+    if rc:
+        raise QmcError(rc, "qmc_dmc_branch")
 
 
 # fixed fields of one reptation trace row before the per-bead probabilities

@@ -32,7 +32,15 @@ reference:
   from jfirst in `<x_file>.res.<k>` after every block (restart(1),
   reptation.f:4481-4490) and the final path to `<x_file>.<k>` (write_path,
   reptation.f:1741-1764); after a `restart` record the trial energy is the
-  cumulative mean energy of the restart file (restart(0), reptation.f:4525).
+  cumulative mean energy of the restart file (restart(0), reptation.f:4525);
+* a `dmc` record (reptation.f:768-782; dmc, reptation.f:872-921) runs
+  diffusion Monte Carlo with branch2's fixed population, each rank holding
+  one configuration (ntarget 1): per step mmstep moves with their weights,
+  the weighted block sums, the branching of every rank's walker; after each
+  block the `===>> dmc block` line with nconf, the population bounds and
+  rank 0's forced moves, the trial energy set to the cumulative mean energy,
+  `.res` (both generators of every rank) and `<x_file>.res.<k>`; at the end
+  `<x_file>.<k>` and the count of weights bounded on rank 0.
 
 Block averages sum p_old over the walkers and steps of the block, the
 MPI_REDUCE of the reference (reptation.f:4308-4311), so that blk_norm is
@@ -256,11 +264,11 @@ class Run:
             f.write(" " + " ".join(str(v) for v in self.limbs[0]) + "\n")
 
     # -- output of a block
-    def block_output(self, iblk, blk, blk_norm, names, info, who="vmc"):
+    def block_output(self, iblk, blk, blk_norm, names, info, who="vmc", extra=""):
         st = self.stats
         err = st.err(iblk)
         w = self.out.write
-        w(f" ===>> {who} block  {iblk}\n")
+        w(f" ===>> {who} block  {iblk}{extra}\n")
         for i, nm in enumerate(names):
             w(fortran_e(blk[i] / blk_norm, 19, 11) + fortran_e(st.av[i] / st.norm, 19, 11)
               + fortran_e(err[i], 9, 2) + fortran_e(blk_norm, 10, 3) + "  " + nm.ljust(20) + "\n")
@@ -375,6 +383,64 @@ class Run:
         finally:
             ctx.close()
 
+    # -- the dmc command
+    def dmc(self, nblk, nstp, delta, ntarget, mmstep, etrial, mage, mmult):
+        import torch
+        d = self.deck
+        if ntarget != 1:
+            raise NotImplementedError("dmc: each rank holds one configuration here (ntarget 1); the "
+                                      "reference's ranks with several share one random stream, "
+                                      "walker after walker")
+        if d.ntheta != 0:
+            raise NotImplementedError("dmc with twist averaging (tabc(dmc)) is not on this path")
+        ctx = hip.Context(d, twist=(1, 1))
+        try:
+            info = ctx.info
+            names = scalar_names(d, info)
+            jetot = info["jetot"] - 1
+            if self.res_string != ".":
+                self.read_restart(nblk, info["n_props"], who="dmc")
+                etrial = self.stats.av[jetot] / self.stats.norm
+            x = self.read_configs()
+            states = [abi.limbs_to_state(l) for l in self.limbs]
+            wk = hip.Walkers(ctx, x, states)
+            wk.start(delta=delta)
+            dm = hip.Dmc(wk, self.limbs2[0])
+            for iblk in range(self.iblk0, nblk + 1):
+                if iblk == 1 or self.stats is None:
+                    self.stats = _Stats(info["n_props"])
+                wk.blk.zero_()
+                dm.wsum.zero_()
+                # averages(1) resets rank 0's count of forced moves
+                dm.counts[3] = 0
+                dm.sweep(nstp, delta, etrial, mmstep=mmstep, mmult=mmult, mage=mage)
+                torch.cuda.synchronize()
+                st = wk.status()
+                if np.any(st & 2):
+                    raise RuntimeError("singular orbital matrix (the reference stops here, reptation.f:10318)")
+                blk = wk.blk.sum(dim=0).cpu().numpy()
+                blk_norm = float(dm.wsum.sum().item())
+                cnt = dm.counts.cpu().numpy()
+                self.stats.add(blk, blk_norm)
+                self.limbs = [abi.state_to_limbs(int(v)) for v in wk.rng.cpu().numpy()]
+                l2 = abi.state_to_limbs(int(dm.rng2.cpu().numpy()[0]))
+                self.limbs2 = [l2 for _ in range(self.nw)]
+                self.out.write(f" ***** mytid 0 nsg =  {_ld(0.0)}\n")
+                # averages(3) for dmc: the trial energy is the cumulative
+                # mean energy (reptation.f:4327-4333); min/max_nconf change
+                # only in branch (dmc1), so they keep averages(1)'s values
+                etrial = self.stats.av[jetot] / self.stats.norm
+                self.block_output(iblk, blk, blk_norm, names, info, who="dmc",
+                                  extra=f" nconf =  1  mnnc, mxnc =  10000000  0  nage =  {_ld(float(cnt[3]))}")
+                self.write_restart(iblk, who="dmc")
+                self.write_configs(wk.x.cpu().numpy())
+            self.write_configs(wk.x.cpu().numpy(), ".")
+            cnt = dm.counts.cpu().numpy()
+            self.out.write(f" max.multiplicity  {mmult}  exceeded  {int(cnt[2])}  times\n")
+            wk = dm = None
+        finally:
+            ctx.close()
+
     def execute(self):
         """Run the records of `<runid>.in`."""
         recs = read_script(self.dir, self.runid)
@@ -406,7 +472,17 @@ class Run:
                 etrial, mage, mage_r = (deckmod._ffloat(v) for v in w[7:10])
                 self.rmc(nblk, nstp, delta, ntau, ntauskip, mdelta, etrial, mage, mage_r)
                 continue
-            raise NotImplementedError(f"{w[0]}: only vmc, rmc, restart and end records run on this path")
+            if w[0] == "dmc":
+                # dmc nblk nstp delta ntarget mmstep etrial mage mmult
+                # (reptation.f:768-777)
+                nblk, nstp = int(w[1]), int(w[2])
+                delta = deckmod._ffloat(w[3])
+                ntarget, mmstep = int(w[4]), int(w[5])
+                etrial, mage = deckmod._ffloat(w[6]), deckmod._ffloat(w[7])
+                mmult = int(w[8])
+                self.dmc(nblk, nstp, delta, ntarget, mmstep, etrial, mage, mmult)
+                continue
+            raise NotImplementedError(f"{w[0]}: only vmc, rmc, dmc, restart and end records run on this path")
         self.write_seed()
 
     def _n_props(self):

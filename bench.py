@@ -143,6 +143,8 @@ def main():
                     help="VMC steps of the plain N=26 workload instead: 13+13, rs=5, u2 only, "
                          "zero backflow, 60 k-shells, 256 walkers; the step replayed as a "
                          "captured hipGraph and launched directly (QMC_NO_GRAPH)")
+    ap.add_argument("--dmc", action="store_true",
+                    help="DMC steps/s with fixed-population branching (N=122, 4096 walkers; N=26, 256)")
     ap.add_argument("--jobs", type=int, default=40)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--max-blocks", type=int, default=1000,
@@ -172,6 +174,8 @@ def main():
         return bench_rqmc(args, hip, deck, dg)
     if args.vmc:
         return bench_vmc(args, hip, deck, dg)
+    if args.dmc:
+        return bench_dmc(args, hip, deck, dg)
     if args.serve:
         return bench_serve(args)
     tmp = tempfile.mkdtemp(prefix="degas_bench_")
@@ -349,6 +353,76 @@ def bench_vmc(args, hip, deck, dg):
         "direct_launch_steps_per_s": res["direct"],
         "graph_speedup": res["graph"] / res["direct"],
         "acceptance_last_step": acc,
+    }
+    print(json.dumps(out))
+
+
+def bench_dmc(args, hip, deck, dg):
+    """DMC steps/s (a move of every walker, its weight, the weighted block
+    sums, the branching and the copies: qmc_dmc_sweep with mmstep 1) on the
+    N=122 workload with 4096 walkers and on the plain N=26 one with 256,
+    and the time of the branching alone (one thread over the walkers)."""
+    import numpy as np
+    import torch
+    abi = importlib.import_module("2degas_amd.abi")
+    res = {}
+    for tag, nu, rs, shells, nw, kw in (("n122", 61, 10.0, 200, 4096, {}),
+                                        ("n26", 13, 5.0, 60, 256, dict(u3=False, backflow="zero"))):
+        tmp = tempfile.mkdtemp(prefix="degas_dmc_")
+        deckdir = os.path.join(tmp, "deck")
+        dg.make_deck(deckdir, tag, nu, nu, rs, shells, **kw)
+        d = deck.read_deck(deckdir)
+        x = dg.lattice_configs(nw, d.nptot, d.el[0], 777)
+        sts = [abi.limbs_to_state([3535, 3467, 3262, (2 * (1549 + k) + 1) % 4096]) for k in range(nw)]
+        ctx = hip.Context(d, twist=(1, 1), device=0)
+        wk = hip.Walkers(ctx, x, sts)
+        delta = 0.05 * rs * rs / d.nptot
+        wk.start(delta=delta)
+        wk.sweep(20, delta)
+        torch.cuda.synchronize()
+        etrial = float(wk.props[:, ctx.info["jetot"] - 1].mean().item())
+        dm = hip.Dmc(wk, [3535, 3467, 3262, 1549])
+        dm.sweep(max(args.warmup, 2), delta, etrial)
+        torch.cuda.synchronize()
+        nsteps = args.steps if tag == "n122" else 10 * args.steps
+        t0 = time.perf_counter()
+        dm.sweep(nsteps, delta, etrial)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        copies = int((dm.src != torch.arange(nw, device=dm.src.device, dtype=torch.int32)).sum().item())
+        # the branching alone, on the last step's weights
+        w0 = torch.rand(nw, dtype=torch.float64, device=wk.x.device) + 0.5
+        wt, r2 = w0.clone(), dm.rng2.clone()
+        s0, c0 = dm.src.clone(), dm.counts.clone()
+        hip.dmc_branch(ctx, wt, r2, s0, c0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(20):
+            wt.copy_(w0)
+            hip.dmc_branch(ctx, wt, r2, s0, c0)
+        torch.cuda.synchronize()
+        tb = (time.perf_counter() - t1) / 20
+        res[tag] = {"steps_per_s": nsteps / dt, "walker_moves_per_s": nsteps * nw / dt, "walkers": nw,
+                    "N": d.nptot, "ms_per_step": dt / nsteps * 1e3, "branch_ms": tb * 1e3,
+                    "copies_last_step": copies}
+        wk = dm = None
+        ctx.close()
+    out = {
+        "metric": "DMC steps/s (N=122, 4096 walkers; move, weights, branching, copies)",
+        "value": res["n122"]["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["n122"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "DMC with fixed-population branching, N=122 (61+61) rs=10 "
+                               "u2+u3+backflow, 200 k-shells, 4096 walkers, mmstep 1"},
+        "workloads": res,
     }
     print(json.dumps(out))
 

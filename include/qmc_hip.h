@@ -323,6 +323,64 @@ int qmc_rmc_sweep(qmc_ctx* ctx, const qmc_reptiles* r, const qmc_rmc_params* prm
                   double* trace, void* stream);
 
 /*
+ * Diffusion Monte Carlo with fixed-population branching (the reference's dmc
+ * driver, reptation.f:872-921, and branch2, reptation.f:2162-2271), each
+ * walker playing one MPI rank that holds one configuration (ntarget 1): its
+ * moves draw its rank's stream, the comb of branch2 draws the second
+ * generator (rannyu2, seeded alike on every rank, reptation.f:36-37), and a
+ * walker whose multiplicity comes out 0 takes a copy of a walker with extra
+ * copies, as the reference's send/receive between ranks does.
+ *
+ * qmc_dmc_params: the dmc record (reptation.f:768-777) for one call.
+ * qmc_dmc_state: device buffers owned by the caller, for nw walkers.
+ */
+typedef struct qmc_dmc_params {
+  double delta;      /* time step                                          */
+  double etrial;     /* trial energy of the branching weights               */
+  double mage;       /* consecutive rejections that force a move            */
+  int32_t mmstep;    /* moves per walker between branchings                 */
+  int32_t mmult;     /* bound of one move's weight factor (peso)            */
+  int32_t bm_order;  /* Box-Muller draw order, as for qmc_vmc_sweep         */
+  int32_t reserved0; /* zero                                                */
+} qmc_dmc_params;
+
+typedef struct qmc_dmc_state {
+  double* wate;     /* [nw] weight of the current step (branch2's wate)     */
+  double* wsum;     /* [nw] sums of the weights of averages(2) (blk_norm)   */
+  int32_t* src;     /* [nw] walker each one was copied from by the last
+                       branching (itself when kept), 0-based              */
+  int64_t* rng2;    /* [1]  state of the second generator                   */
+  int32_t* counts;  /* [4]  added to: weight factors bounded at mmult (all
+                       walkers), weights bounded at 10 (all walkers; branch2's
+                       nmult), of them walker 0's (rank 0's nmult), moves of
+                       walker 0 forced by mage (rank 0's nage)             */
+} qmc_dmc_state;
+
+/*
+ * nsteps DMC steps for every walker: mmstep times move + metropolis_test
+ * (as qmc_vmc_sweep), the local energy of the step (the mean of the old and
+ * new energies when accepted, reptation.f:2631), the weight factor
+ * peso = exp(-npnorm (elocal - etrial) delta) bounded at mmult, wate *= peso
+ * and averages(2) with weight wate (w->blk += p_old wate, d->wsum += wate;
+ * w->blk must be set); then branch2 as qmc_dmc_branch and the copies.
+ * d->wate is set to 1 at the start of every step.
+ */
+int qmc_dmc_sweep(qmc_ctx* ctx, const qmc_walkers* w, const qmc_dmc_state* d, int nsteps,
+                  const qmc_dmc_params* prm, void* stream);
+
+/*
+ * branch2 alone on nw weights (device): weights above 10 are set to 10
+ * (counts[1], and counts[2] for weight 0), the comb of nint(sum of the
+ * fractional parts) rannyu2 draws on the normalized weights gives each
+ * walker its multiplicity, and the walkers with multiplicity 0, in
+ * ascending order, are given the extra copies of the walkers with more
+ * than one, in ascending order: src[j] is where walker j's next
+ * configuration comes from.  d_wate is overwritten.
+ */
+int qmc_dmc_branch(qmc_ctx* ctx, int nw, double* d_wate, int64_t* d_rng2, int32_t* d_src,
+                   int32_t* d_counts, void* stream);
+
+/*
  * Per-kernel timing with HIP events recorded on the launch stream around each
  * kernel of every evaluation while enabled (no counterpart in the reference,
  * which has no timers).  qmc_timing(ctx, 1) resets and enables,

@@ -170,6 +170,71 @@ def rank_seed_limbs(seed, rank):
 REF_DIR = os.path.join(HERE, "_ref")
 
 
+_LCG_A = 34522712143931
+_MASK48 = (1 << 48) - 1
+
+
+def branch2(wate, limbs2):
+    """branch2 (reptation.f:2162-2271) on one rank holding the walkers in
+    slots 1..n, restated: weights bounded at wmult = 10 (reptation.f:2176-2182),
+    normalized to the population in slot order, integer parts kept and the
+    fractional parts combed with nint(sum) draws of the second generator
+    (reptation.f:2189-2210, the LCG x <- a x mod 2^48 of rannyu2,
+    reptation.f:5356-5399), then every slot with multiplicity 0, in
+    ascending order, takes a copy from the lowest slot with more than one
+    (prov_dest, reptation.f:2258-2271).
+    Returns (src, nmult, limbs2 after, mult): src[j] is the 0-based slot
+    slot j's configuration comes from."""
+    w = np.array(wate, dtype=np.float64)
+    n = len(w)
+    nmult = int(np.sum(w > 10.0))
+    w = np.where(w > 10.0, 10.0, w)
+    somma = 0.0
+    for v in w:
+        somma = somma + float(v)
+    tm = np.zeros(n, dtype=np.int64)
+    dl = np.zeros(n)
+    k0 = 0.0
+    for i in range(n):
+        ww = (float(n) * float(w[i])) / somma
+        tm[i] = int(ww)
+        dl[i] = ww - float(tm[i])
+        k0 = k0 + dl[i]
+    x = abi.limbs_to_state(limbs2)
+    j, k = 0, 1.0
+    ndraw = int(np.floor(k0 + 0.5)) if k0 >= 0 else 0
+    for _ in range(ndraw):
+        x = (_LCG_A * x) & _MASK48
+        r = x * 2.0 ** -48
+        k = k - 1.0
+        while k < r and j < n:
+            j += 1
+            k = k + dl[j - 1]
+        tm[j - 1] += 1
+        while k < 1.0 and j < n:
+            j += 1
+            k = k + dl[j - 1]
+    mult = tm.copy()
+    src = np.arange(n)
+    m = tm.copy()
+    for jj in range(n):
+        if m[jj] == 0:
+            for i in range(n):
+                if m[i] >= 2:
+                    src[jj] = i
+                    m[i] -= 1
+                    m[jj] += 1
+                    break
+    return src, nmult, abi.state_to_limbs(x), mult
+
+
+def parse_branch(raw, n):
+    """Mode 5 output: the slot origins (1-based), nmult and the second
+    generator's limbs."""
+    a = np.frombuffer(raw, dtype=np.int32)[5:]
+    return {"src": a[:n] - 1, "nmult": int(a[n]), "limbs2": [int(v) for v in a[n + 1:n + 5]]}
+
+
 def have_ref():
     return os.path.exists(os.path.join(REF_DIR, "refeval"))
 

@@ -20,6 +20,14 @@ c
 c Mode 4 only, line 7:
 c   ntau ntauskip mdelta etrial mage mage_r every
 c
+c Mode 5 follows branch2 (reptation.f:2162-2271) on one rank: ncfg
+c walkers in stack slots 1..ncfg with the weights of cfgfile (raw
+c float64, ncfg values), the second generator seeded by input from the
+c deck's seed file; slot i's first coordinate holds i, so that the
+c redistribution (prov_dest) shows where each slot's configuration came
+c from.  Output: the slot origins, nmult (weights clamped at wmult) and
+c the second generator's limbs after the draws.
+c
 c Mode 1 follows read_conf (reptation.f:2452-2502) without the stack:
 c x_new <- config; compute_properties(1) (reptation.f:2661).
 c Mode 2 follows vmc (reptation.f:973-1017): one initial
@@ -39,6 +47,8 @@ c state is dumped before the move.
       integer*8 nrate,t0,t1
       parameter(nbuf=64)
       real*8 p,del,xb(2,mnp,nbuf),wnew,retrial,rmage,rmage_r,pb(64)
+      real*8 wb(mstack)
+      integer mb(mstack)
       character*256 cfgfile,outfile
       mytid=0
       nproc=1
@@ -171,6 +181,20 @@ c direction, beads attempted, path weight, Langevin p of each bead
        enddo
        call savern(seedv)
        write(13) seedv,jfirst,jlast
+      elseif(mode.eq.5)then
+       do i=1,ncfg
+        read(12) wb(i)
+        x_stack(1,1,i)=dble(i)
+       enddo
+       getnext=1
+       putnext=mod(ncfg,mstack)+1
+       nstack=ncfg
+       nconf=ncfg
+       nmult=0
+       call branch2(wb,mb)
+       write(13) (nint(x_stack(1,1,i)),i=1,ncfg)
+       call savern2(seedv)
+       write(13) nmult,seedv
       endif
       close(12)
       close(13)

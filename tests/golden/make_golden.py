@@ -27,9 +27,10 @@ All generated decks come from the restated generators of the reference
            gstorto + nodalaction, wpiu, rejection 0, ecut with a value (with
            the drift cutoff on, without which ecut changes nothing)
 
-Usage: python tests/golden/make_golden.py [rmc|rmcvar|adrift]   (needs
-       oracle/_ref; minutes; "rmc" writes only the reptation fixtures,
-       "rmcvar" the propagator variants, "adrift" the drift-cutoff ones)
+Usage: python tests/golden/make_golden.py [rmc|rmcvar|adrift|dmcbranch]
+       (needs oracle/_ref; minutes; "rmc" writes only the reptation
+       fixtures, "rmcvar" the propagator variants, "adrift" the drift-cutoff
+       ones, "dmcbranch" the DMC branching ones)
 """
 import importlib
 import os
@@ -197,6 +198,38 @@ def make_adrift():
     replay_fixture("n26a", 1000, 100, 0.02, shipped[0], tag="n26a_d02")
 
 
+def make_dmc_branch():
+    """branch2 of the reference (refeval mode 5, one rank) on weight sets:
+    populations 1 to 1000, log-normal weights of three spreads, weights
+    above the bound 10, equal weights (nothing to comb) and a few very
+    small ones; two seeds."""
+    rng = np.random.default_rng(31)
+    cases = []
+    for n in (1, 2, 7, 20, 64, 257, 1000):
+        for sd in (0.1, 0.6, 1.5):
+            cases.append(rng.lognormal(0.0, sd, size=n))
+    cases.append(np.full(50, 0.7))
+    w = rng.lognormal(0.0, 0.3, size=100)
+    w[[3, 50, 99]] = [12.0, 40.0, 10.5]
+    cases.append(w)
+    w = rng.lognormal(0.0, 0.3, size=40)
+    w[::7] = 1e-9
+    cases.append(w)
+    seeds = ([3535, 3467, 3262, 1549], [11, 2222, 33, 4001])
+    out = {}
+    for c, w in enumerate(cases):
+        seed = seeds[c % 2]
+        raw, _ = po.run_refeval(os.path.join(DECKS, "n26"), os.path.join(WORK, "br"), 5, w, seed=seed)
+        r = po.parse_branch(raw, len(w))
+        out[f"w_{c}"] = w
+        out[f"seed_{c}"] = np.array(seed)
+        out[f"src_{c}"] = r["src"]
+        out[f"nmult_{c}"] = np.array(r["nmult"])
+        out[f"limbs2_{c}"] = np.array(r["limbs2"])
+    out["ncases"] = np.array(len(cases))
+    np.savez_compressed(os.path.join(HERE, "dmc_branch.npz"), **out)
+
+
 def main():
     if not po.have_ref():
         sys.exit("oracle/_ref missing: run oracle/build_ref.sh first")
@@ -246,6 +279,8 @@ if __name__ == "__main__":
         make_rmc()
     elif sys.argv[1:] == ["adrift"]:
         make_adrift()
+    elif sys.argv[1:] == ["dmcbranch"]:
+        make_dmc_branch()
     elif sys.argv[1:] == ["rmcvar"]:
         shipped = deck.read_configs([os.path.join(DECKS, "n26", f) for f in
                                      ("n26.up.x.res.0", "n26.down.x.res.0")], [13, 13])
@@ -254,3 +289,4 @@ if __name__ == "__main__":
         main()
         make_rmc()
         make_adrift()
+        make_dmc_branch()

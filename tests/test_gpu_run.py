@@ -366,3 +366,55 @@ def test_reference_rmc_driver_with_gpu_compute_properties(tmp_path):
         b = np.loadtxt(os.path.join(cpu, f"n26.{t}.x.0"))
         assert a.shape == b.shape
         assert np.max(np.abs(a - b)) < 1e-8
+
+
+DMC_REC = "dmc {nblk} 15 0.02 1 2 -0.44 1000000 10\n"
+
+
+def _blocks_dmc(text):
+    """dmc block lines: the header's numbers (nconf, the population bounds,
+    nage) per block, and the property lines as _blocks reads them."""
+    hdr, lines = {}, []
+    for ln in text.splitlines():
+        if "===>> dmc block" in ln:
+            t = ln.split()
+            b = int(t[3])
+            hdr[b] = [float(v.replace("D", "E")) for v in (t[6], t[10], t[11], t[14])]
+            lines.append(f" ===>> vmc block {b}")
+        else:
+            lines.append(ln)
+    return _blocks("\n".join(lines)), hdr
+
+
+@NEEDS_REF
+def test_dmc_one_walker_reproduces_reference_driver_files(tmp_path):
+    """A dmc record (reptation.f:768-782; dmc, reptation.f:872-921) with one
+    walker on one rank (ntarget 1, so branch2 keeps it): every block's line
+    (the weighted averages of every property, the population bounds, nage),
+    the trial energy it sets, `.res` sums and both generators' limbs, the
+    configurations after each block and at the end, and the count of bounded
+    weights."""
+    print("\n[gpu] batched dmc driver vs reference driver, 1 walker", flush=True)
+    ref, ours = str(tmp_path / "ref"), str(tmp_path / "ours")
+    for w in (ref, ours):
+        _deck_dir(w)
+        _script(w, DMC_REC.format(nblk=3) + "end\n")
+    out_r, out_o = _ref(ref), _ours(ours, 1)
+    (br, hr), (bo, ho) = _blocks_dmc(out_r), _blocks_dmc(out_o)
+    assert sorted(br) == sorted(bo) == [1, 2, 3]
+    for b in br:
+        assert [n for _, n in br[b]] == [n for _, n in bo[b]]
+        assert hr[b] == ho[b], f"block {b} header {hr[b]} vs {ho[b]}"
+        _close_lines(bo[b], br[b], 1e-8, f"dmc block {b}")
+    tr, to = _res(ref), _res(ours)
+    assert tr[:2] == to[:2] == ["3", "dmc"] and len(tr) == len(to)
+    _close([float(v) for v in to[2:-8]], [float(v) for v in tr[2:-8]], 1e-8, "restart sums")
+    assert tr[-8:] == to[-8:]
+    for t in TYPES:
+        for mid in (".res.", "."):
+            a = np.loadtxt(os.path.join(ours, f"n26.{t}.x{mid}0"))
+            r = np.loadtxt(os.path.join(ref, f"n26.{t}.x{mid}0"))
+            assert a.shape == r.shape and np.max(np.abs(a - r)) < 1e-8
+    mr = [ln for ln in out_r.splitlines() if "max.multiplicity" in ln]
+    mo = [ln for ln in out_o.splitlines() if "max.multiplicity" in ln]
+    assert len(mr) == len(mo) == 1 and mr[0].split()[-2] == mo[0].split()[-2]

@@ -180,3 +180,39 @@ def test_rel_err_rejects_nan():
     assert rel_err([1.0], [np.nan]) == float("inf")
     assert rel_err([np.nan], [np.nan], nan_ok=True) == 0.0
     assert rel_err([1.0, np.nan], [1.0, 2.0], nan_ok=True) == float("inf")
+
+
+def test_branch2_restatement_matches_reference_and_combs(oracle_mod):
+    """The oracle's branch2 against the reference's (refeval mode 5) on every
+    golden weight set: the same origin for every slot, bounded-weight count
+    and second-generator state.  And what the comb must give whatever its
+    code: it draws one independent uniform in each unit of the running sum
+    of the fractional parts (reptation.f:2196-2210, a stratified comb), so
+    every multiplicity is the normalized weight's integer part plus the
+    number of those units its fractional part was picked in (at most the
+    units its stretch of the running sum meets, at most 2); they add up to
+    the population; each slot's configuration comes from a slot with
+    multiplicity at least 1 (itself when kept)."""
+    po = oracle_mod
+    run = pkg("run")
+    g = load_golden("dmc_branch.npz")
+    for c in range(int(g["ncases"])):
+        w = np.asarray(g[f"w_{c}"])
+        limbs2 = run.setrn_limbs([int(v) for v in g[f"seed_{c}"]])
+        src, nmult, l2, mult = po.branch2(w, limbs2)
+        assert np.array_equal(src, g[f"src_{c}"]), c
+        assert nmult == int(g[f"nmult_{c}"]) and l2 == [int(v) for v in g[f"limbs2_{c}"]], c
+        n = len(w)
+        wb = np.minimum(w, 10.0)
+        ww = n * wb / wb.sum()
+        fl = np.floor(ww)
+        cum = np.concatenate([[0.0], np.cumsum(ww - fl)])
+        # units [i-1, i) of the running sum each stretch [cum_j, cum_j+1) meets
+        meets = np.ceil(cum[1:] - 1e-12) - np.floor(cum[:-1] + 1e-12)
+        extra = mult - fl
+        assert mult.sum() == n
+        assert np.all(extra >= 0) and np.all(extra <= np.minimum(meets, 2)), c
+        kept = mult >= 1
+        assert np.all(src[kept] == np.arange(n)[kept])
+        assert np.all(mult[src] >= 1)
+        assert np.array_equal(np.bincount(src, minlength=n), mult)
