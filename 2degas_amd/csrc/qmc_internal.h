@@ -79,6 +79,9 @@ struct qmc_ctx {
   bool use_graph = true;
   hipGraphExec_t vmc_exec = nullptr;
   std::vector<uint64_t> vmc_key;
+  // the reptation move as a captured graph, and what it was captured with
+  hipGraphExec_t rmc_exec = nullptr;
+  std::vector<uint64_t> rmc_key;
   // reptation scratch for rmc_nr reptiles: move draws and per-move sums
   int rmc_nr = 0;
   int32_t* d_rmv = nullptr;

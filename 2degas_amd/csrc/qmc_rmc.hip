@@ -15,6 +15,8 @@
 // below idelta is neither evaluated nor draws.
 #include "qmc_internal.h"
 
+#include <cstring>
+
 enum { RW_BRA = 0, RW_DTP = 1, RW_ITP = 2, RW_LNG = 3, RW_NDS = 4, RW_N = 5 };
 #define RMC_TR 8  // fixed trace fields before the per-bead probabilities
 
@@ -542,7 +544,8 @@ static int rmc_setup(qmc_ctx* c, const qmc_reptiles* r, const qmc_rmc_params* pr
       return arg_fail("reptation needs every populated type to move (hbs2m != 0)");
   // the rmc record sets the COMMON delta before sigma (reptation.f:795-805)
   c->delta = prm->delta;
-  k = RmcK{};
+  // every byte set (padding included): the captured move is keyed on them
+  memset(&k, 0, sizeof k);
   k.nr = r->nr;
   k.ntau = r->ntau;
   k.nbuf = r->nbuf;
@@ -633,35 +636,88 @@ extern "C" int qmc_rmc_sweep(qmc_ctx* c, const qmc_reptiles* r, const qmc_rmc_pa
   if (rc) return rc;
   rc = ensure_rmc(c, r->nr);
   if (rc) return rc;
+  rc = ensure_ws(c, r->nr);
+  if (rc) return rc;
   const int nr = r->nr;
   double* sc = c->d_rsc;
   double* g2n = c->d_rsc + 3 * nr;
   double* ppn = c->d_rsc + 4 * nr;
   const int wpb = QMC_BLOCK / 64;
   const int64_t rowlen = RMC_TR + prm->mdelta;
-  for (int s = 0; s < nsteps; s++) {
-    double* tr = trace ? trace + (int64_t)s * nr * rowlen : nullptr;
-    hipLaunchKernelGGL(k_rmc_begin, dim3((nr + 63) / 64), dim3(64), 0, st, k, r->ends, r->rng,
-                       c->d_rmv);
-    hipLaunchKernelGGL(k_rmc_slots, dim3(1), dim3(RMC_SLOT_THREADS), 0, st, k, c->d_rmv, c->d_rslot,
+  // one move of every reptile on stream q (trace row tr, or none)
+  auto move = [&](hipStream_t q, double* tr) -> int {
+    hipLaunchKernelGGL(k_rmc_begin, dim3((nr + 63) / 64), dim3(64), 0, q, k, r->ends, r->rng, c->d_rmv);
+    hipLaunchKernelGGL(k_rmc_slots, dim3(1), dim3(RMC_SLOT_THREADS), 0, q, k, c->d_rmv, c->d_rslot,
                        c->d_rslot + nr);
     for (int id = 1; id <= prm->mdelta; id++) {
       hipLaunchKernelGGL(k_rmc_propose, dim3((nr + wpb - 1) / wpb), dim3(QMC_BLOCK),
-                         sizeof(double) * 3 * 2 * k.N * wpb, st, k, id, r->x, r->g, c->d_rmv,
+                         sizeof(double) * 3 * 2 * k.N * wpb, q, k, id, r->x, r->g, c->d_rmv,
                          c->d_rslot, r->rng, c->d_xn, sc, r->grad2_ph, g2n);
       // only the reptiles moving bead id are evaluated (slots 0..nact-1)
-      rc = launch_eval(c, nr, c->d_xn, c->d_pn, c->d_gn, c->d_hn, g2n, c->d_stn, st, prm->delta,
-                       QMC_PREC_FP64, c->d_rslot + nr + id - 1);
-      if (rc) return rc;
-      hipLaunchKernelGGL(k_rmc_step, dim3((nr + wpb - 1) / wpb), dim3(QMC_BLOCK), 0, st, k, id,
-                         r->x, r->g, r->props, r->age, r->w, c->d_rmv, c->d_rslot, r->rng, c->d_xn,
-                         c->d_gn, c->d_pn, sc, r->grad2_ph, g2n, c->d_stn, c->d_sticky, tr);
+      const int rc2 = launch_eval(c, nr, c->d_xn, c->d_pn, c->d_gn, c->d_hn, g2n, c->d_stn, q, prm->delta,
+                                  QMC_PREC_FP64, c->d_rslot + nr + id - 1);
+      if (rc2) return rc2;
+      hipLaunchKernelGGL(k_rmc_step, dim3((nr + wpb - 1) / wpb), dim3(QMC_BLOCK), 0, q, k, id, r->x, r->g,
+                         r->props, r->age, r->w, c->d_rmv, c->d_rslot, r->rng, c->d_xn, c->d_gn, c->d_pn, sc,
+                         r->grad2_ph, g2n, c->d_stn, c->d_sticky, tr);
     }
-    hipLaunchKernelGGL(k_rmc_path, dim3(nr, (k.NP + 63) / 64), dim3(64), 0, st, k, r->props,
-                       r->ends, c->d_rmv, ppn);
-    hipLaunchKernelGGL(k_rmc_finish, dim3(nr), dim3(64), 0, st, k, r->props, r->w, r->ends,
-                       c->d_rmv, r->rng, r->rage, r->pp_old, ppn, r->blk, tr);
+    hipLaunchKernelGGL(k_rmc_path, dim3(nr, (k.NP + 63) / 64), dim3(64), 0, q, k, r->props, r->ends,
+                       c->d_rmv, ppn);
+    hipLaunchKernelGGL(k_rmc_finish, dim3(nr), dim3(64), 0, q, k, r->props, r->w, r->ends, c->d_rmv,
+                       r->rng, r->rage, r->pp_old, ppn, r->blk, tr);
+    return QMC_OK;
+  };
+  const bool graph = c->use_graph && !trace && !c->timing && nsteps > 1;
+  if (!graph) {
+    for (int s = 0; s < nsteps; s++) {
+      rc = move(st, trace ? trace + (int64_t)s * nr * rowlen : nullptr);
+      if (rc) return rc;
+    }
+    HIPCHK(hipGetLastError());
+    return QMC_OK;
   }
+  // The move as a captured graph, replayed nsteps times, as the VMC step:
+  // rebuilt whenever anything its kernels were given changes (buffers, the
+  // move's parameters, the g-vectors).  Captured on the context's own
+  // stream, launched on st.
+  std::vector<uint64_t> key = {(uint64_t)nr,
+                               (uint64_t)(uintptr_t)r->x,       (uint64_t)(uintptr_t)r->g,
+                               (uint64_t)(uintptr_t)r->props,   (uint64_t)(uintptr_t)r->age,
+                               (uint64_t)(uintptr_t)r->w,       (uint64_t)(uintptr_t)r->ends,
+                               (uint64_t)(uintptr_t)r->rage,    (uint64_t)(uintptr_t)r->pp_old,
+                               (uint64_t)(uintptr_t)r->grad2_ph, (uint64_t)(uintptr_t)r->rng,
+                               (uint64_t)(uintptr_t)r->blk,     (uint64_t)(uintptr_t)c->d_xn,
+                               (uint64_t)(uintptr_t)c->d_rmv,   (uint64_t)(uintptr_t)c->d_rslot,
+                               (uint64_t)(uintptr_t)c->d_rsc,   (uint64_t)(uintptr_t)c->d_ws,
+                               (uint64_t)(uintptr_t)c->d_F,     (uint64_t)(uintptr_t)c->d_A,
+                               (uint64_t)(uintptr_t)c->s.gvec,  (uint64_t)c->s.ng};
+  {
+    uint64_t words[(sizeof(RmcK) + 7) / 8] = {};
+    memcpy(words, &k, sizeof k);
+    for (uint64_t wd : words) key.push_back(wd);
+  }
+  if (!c->rmc_exec || key != c->rmc_key) {
+    if (c->rmc_exec) (void)hipGraphExecDestroy(c->rmc_exec);
+    c->rmc_exec = nullptr;
+    c->rmc_key.clear();
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = move(c->stream, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    const hipError_t e2 = hipGraphInstantiate(&c->rmc_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+      c->rmc_exec = nullptr;
+      return hip_fail(e2, "hipGraphInstantiate");
+    }
+    c->rmc_key = key;
+  }
+  for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->rmc_exec, st));
   HIPCHK(hipGetLastError());
   return QMC_OK;
 }

@@ -247,6 +247,9 @@ void free_vmc_graph(qmc_ctx* c) {
   if (c->vmc_exec) (void)hipGraphExecDestroy(c->vmc_exec);
   c->vmc_exec = nullptr;
   c->vmc_key.clear();
+  if (c->rmc_exec) (void)hipGraphExecDestroy(c->rmc_exec);
+  c->rmc_exec = nullptr;
+  c->rmc_key.clear();
 }
 
 extern "C" int qmc_vmc_sweep(qmc_ctx* c, const qmc_walkers* w, int nsteps, double delta,

@@ -166,3 +166,43 @@ def test_rmc_block_sums_accumulate_path_averages():
     blk = rep.blk.cpu().numpy()[:, jetot]
     ref = tr.cpu().numpy()[:, :, 4].sum(axis=0)
     assert np.allclose(blk, ref, rtol=1e-13, atol=0)
+
+
+def test_rmc_graph_replays_match_direct_launches():
+    """qmc_rmc_sweep without a trace replays the move as a captured hipGraph;
+    a context made with QMC_NO_GRAPH launches every kernel directly.  From the
+    same reptiles and streams, each replayed sweep leaves exactly the state
+    of the direct one (beads, drifts, properties, ages, link weights, ends,
+    path averages, generator states, block sums), with eager evaluations
+    between the sweeps and after a change of time step (a new capture)."""
+    print("\n[gpu] rmc graph replays vs direct launches", flush=True)
+    import os
+    import torch
+    hip, gold, d, _, prm = _setup("n26r")
+    abi, dg = pkg("abi"), pkg("deckgen")
+    ntau, md = int(gold["ntau"]), int(gold["mdelta"])
+    nr = 8
+    sts = [abi.limbs_to_state([3535, 3467, 3262, (3099 + 2 * k) % 4096]) for k in range(nr)]
+    ctx_g = hip.Context(d, twist=(1, 1))
+    os.environ["QMC_NO_GRAPH"] = "1"
+    try:
+        ctx_d = hip.Context(d, twist=(1, 1))
+    finally:
+        del os.environ["QMC_NO_GRAPH"]
+    beads = np.stack([gold["beads"]] * nr)
+    rg, rd = hip.Reptiles(ctx_g, nr, ntau, md, sts), hip.Reptiles(ctx_d, nr, ntau, md, sts)
+    rg.start(beads, prm)
+    rd.start(beads, prm)
+    other = dg.random_configs(4, d.nptot, d.el[0], 6)
+    prm2 = hip.rmc_params(float(gold["delta"]) * 0.5, float(gold["etrial"]), float(gold["mage"]),
+                          float(gold["mage_r"]), int(gold["ntauskip"]), md, d.driver)
+    for rep, p in enumerate((prm, prm, prm2, prm2)):
+        rg.sweep(5, p)
+        rd.sweep(5, p)
+        torch.cuda.synchronize()
+        for name in ("x", "g", "props", "age", "w", "ends", "rage", "pp_old", "rng", "blk", "grad2"):
+            a, b = getattr(rg, name).cpu().numpy(), getattr(rd, name).cpu().numpy()
+            assert np.array_equal(a, b), f"replay {rep}: {name} differs"
+        ctx_g.eval(other)
+    assert np.all(rg.status() == 0)
+    print("   4 sweeps of 5 moves identical", flush=True)
