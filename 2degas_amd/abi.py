@@ -173,6 +173,14 @@ class QmcDmcState(C.Structure):
     ]
 
 
+class QmcSemState(C.Structure):
+    _fields_ = [
+        ("ainv", C.c_void_p),
+        ("dlogpsi", C.c_void_p),
+        ("nacc", C.c_void_p),
+    ]
+
+
 def limbs_to_state(limbs):
     """48-bit LCG state from the four 12-bit limbs savern writes."""
     l1, l2, l3, l4 = (int(v) for v in limbs)

@@ -55,6 +55,10 @@ module qmc_hip_mod
      type(c_ptr) :: wate, wsum, src, rng2, counts
   end type qmc_dmc_state
 
+  type, bind(C) :: qmc_sem_state
+     type(c_ptr) :: ainv, dlogpsi, nacc
+  end type qmc_sem_state
+
   type, bind(C) :: qmc_rmc_params
      real(c_double) :: delta, etrial, mage, mage_r
      integer(c_int32_t) :: ntauskip, mdelta, rejection, wpiu, gstorto, nodalaction, ecut, bm_order
@@ -162,6 +166,25 @@ module qmc_hip_mod
        type(c_ptr), value :: ctx, wate, rng2, src, counts, stream
        integer(c_int), value :: nw
      end function qmc_dmc_branch
+
+     integer(c_int) function qmc_sem_refresh(ctx, w, d, precision, stream) &
+          bind(C, name='qmc_sem_refresh')
+       import :: c_int, c_ptr, qmc_walkers, qmc_sem_state
+       type(c_ptr), value :: ctx, stream
+       type(qmc_walkers), intent(in) :: w
+       type(qmc_sem_state), intent(in) :: d
+       integer(c_int), value :: precision
+     end function qmc_sem_refresh
+
+     integer(c_int) function qmc_sem_moves(ctx, w, d, j0, j1, sigma, precision, stream) &
+          bind(C, name='qmc_sem_moves')
+       import :: c_int, c_ptr, c_double, qmc_walkers, qmc_sem_state
+       type(c_ptr), value :: ctx, stream
+       type(qmc_walkers), intent(in) :: w
+       type(qmc_sem_state), intent(in) :: d
+       integer(c_int), value :: j0, j1, precision
+       real(c_double), value :: sigma
+     end function qmc_sem_moves
 
      integer(c_int) function qmc_rmc_start(ctx, r, prm, stream) bind(C, name='qmc_rmc_start')
        import :: c_int, c_ptr, qmc_reptiles, qmc_rmc_params

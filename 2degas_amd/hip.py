@@ -19,7 +19,7 @@ EXPORTS = ("qmc_ctx_create", "qmc_ctx_destroy", "qmc_ctx_info", "qmc_set_twist",
            "qmc_last_error", "qmc_flops_per_eval", "qmc_timing", "qmc_timing_read",
            "qmc_vmc_start", "qmc_vmc_sweep", "qmc_vmc_status", "qmc_rmc_start",
            "qmc_rmc_sweep", "qmc_set_delta", "qmc_eval_batch_device_noalloc", "qmc_dmc_sweep",
-           "qmc_dmc_branch")
+           "qmc_dmc_branch", "qmc_sem_refresh", "qmc_sem_moves")
 
 _lib = None
 
@@ -75,6 +75,9 @@ def lib():
         L.qmc_dmc_sweep.argtypes = [vp, C.POINTER(abi.QmcWalkers), C.POINTER(abi.QmcDmcState), C.c_int,
                                     C.POINTER(abi.QmcDmcParams), vp]
         L.qmc_dmc_branch.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
+        L.qmc_sem_refresh.argtypes = [vp, C.POINTER(abi.QmcWalkers), C.POINTER(abi.QmcSemState), C.c_int, vp]
+        L.qmc_sem_moves.argtypes = [vp, C.POINTER(abi.QmcWalkers), C.POINTER(abi.QmcSemState), C.c_int,
+                                    C.c_int, C.c_double, C.c_int, vp]
         _lib = L
     return _lib
 
@@ -307,6 +310,51 @@ def dmc_branch(ctx, wate, rng2, src, counts, stream=None):
                               C.c_void_p(st.cuda_stream))
     if rc:
         raise QmcError(rc, "qmc_dmc_branch")
+
+
+class SingleMoves:
+    """Single-electron moves over a Walkers set (qmc_sem_state): the inverse
+    Slater matrices ([nw][2][mm][mm] complex, float64 or, with
+    precision=abi.QMC_PREC_MIXED, float32), the accumulated log|psi| change and the
+    accepted-move count per walker."""
+
+    def __init__(self, walkers, precision=0):
+        import torch
+        self.wk = walkers
+        self.precision = int(precision)
+        ctx = walkers.ctx
+        np_ = [int(v) for v in ctx.deck.np[:ctx.deck.ntypes]]
+        self.mm = max(np_)
+        nw, dev = walkers.nw, walkers.x.device
+        rt = torch.float32 if self.precision == abi.QMC_PREC_MIXED else torch.float64
+        self.ainv = torch.zeros((nw, 2, self.mm, self.mm, 2), dtype=rt, device=dev)
+        self.dlogpsi = torch.zeros(nw, dtype=torch.float64, device=dev)
+        self.nacc = torch.zeros(nw, dtype=torch.float64, device=dev)
+        s = abi.QmcSemState()
+        s.ainv, s.dlogpsi, s.nacc = self.ainv.data_ptr(), self.dlogpsi.data_ptr(), self.nacc.data_ptr()
+        self.s = s
+
+    def refresh(self, stream=None):
+        """The inverses from the walkers' current positions (qmc_sem_refresh)."""
+        rc = lib().qmc_sem_refresh(self.wk.ctx.h, C.byref(self.wk.s), C.byref(self.s), self.precision,
+                                   self.wk._stream(stream))
+        if rc:
+            raise QmcError(rc, "qmc_sem_refresh")
+
+    def moves(self, j0, j1, sigma, stream=None):
+        """Moves of electrons j0..j1-1 in order (qmc_sem_moves)."""
+        rc = lib().qmc_sem_moves(self.wk.ctx.h, C.byref(self.wk.s), C.byref(self.s), int(j0), int(j1),
+                                 float(sigma), self.precision, self.wk._stream(stream))
+        if rc:
+            raise QmcError(rc, "qmc_sem_moves")
+
+    def sweep(self, sigma, stream=None):
+        """One move of every electron, one spin per launch."""
+        d = self.wk.ctx.deck
+        bounds = [0] + [int(sum(d.np[:t + 1])) for t in range(d.ntypes)]
+        for t in range(d.ntypes):
+            if bounds[t + 1] > bounds[t]:
+                self.moves(bounds[t], bounds[t + 1], sigma, stream)
 
 
 # fixed fields of one reptation trace row before the per-bead probabilities

@@ -145,6 +145,10 @@ def main():
                          "captured hipGraph and launched directly (QMC_NO_GRAPH)")
     ap.add_argument("--dmc", action="store_true",
                     help="DMC steps/s with fixed-population branching (N=122, 4096 walkers; N=26, 256)")
+    ap.add_argument("--sem", action="store_true",
+                    help="single-electron moves with Sherman-Morrison updates instead: plain "
+                         "Slater-Jastrow N=122 (61+61, rs=10, Pade u2, zero backflow, 200 k-shells) "
+                         "with 4096 walkers and N=26 with 256, FP64 and mixed inverses")
     ap.add_argument("--jobs", type=int, default=40)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--max-blocks", type=int, default=1000,
@@ -176,6 +180,8 @@ def main():
         return bench_vmc(args, hip, deck, dg)
     if args.dmc:
         return bench_dmc(args, hip, deck, dg)
+    if args.sem:
+        return bench_sem(args, hip, deck, dg)
     if args.serve:
         return bench_serve(args)
     tmp = tempfile.mkdtemp(prefix="degas_bench_")
@@ -422,6 +428,79 @@ def bench_dmc(args, hip, deck, dg):
         "data": "synthetic",
         "config": {"workload": "DMC with fixed-population branching, N=122 (61+61) rs=10 "
                                "u2+u3+backflow, 200 k-shells, 4096 walkers, mmstep 1"},
+        "workloads": res,
+    }
+    print(json.dumps(out))
+
+
+def bench_sem(args, hip, deck, dg):
+    """Single-electron sweeps/s (qmc_sem_moves over every electron, one spin
+    per launch) with FP64 and FP32 inverses, beside the inverse refresh, the
+    full evaluation that gives a sweep's local energy, and one all-electron
+    VMC step on the same deck and walkers."""
+    import torch
+    abi = importlib.import_module("2degas_amd.abi")
+    res = {}
+    for tag, nu, rs, shells, nw, sigma in (("n122", 61, 10.0, 200, 4096, 0.6), ("n26", 13, 5.0, 60, 256, 0.5)):
+        tmp = tempfile.mkdtemp(prefix="degas_sem_")
+        deckdir = os.path.join(tmp, "deck")
+        dg.make_deck(deckdir, tag, nu, nu, rs, shells, u3=False, backflow="zero", jastrow="pade")
+        d = deck.read_deck(deckdir)
+        x = dg.lattice_configs(nw, d.nptot, d.el[0], 99)
+        sts = [abi.limbs_to_state([3535, 3467, 3262, (2 * (1549 + k) + 1) % 4096]) for k in range(nw)]
+        ctx = hip.Context(d, twist=(1, 1), device=0)
+        wk = hip.Walkers(ctx, x, sts)
+        wk.start()
+        r = {"N": d.nptot, "walkers": nw, "sigma": sigma}
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+        def timed(fn, n):
+            fn()
+            torch.cuda.synchronize()
+            ev[0].record()
+            for _ in range(n):
+                fn()
+            ev[1].record()
+            torch.cuda.synchronize()
+            return ev[0].elapsed_time(ev[1]) / n
+
+        nsw = max(args.steps, 5)
+        for prec, name in ((abi.QMC_PREC_FP64, "fp64"), (abi.QMC_PREC_MIXED, "mixed")):
+            sm = hip.SingleMoves(wk, precision=prec)
+            sm.refresh()
+            for _ in range(max(args.warmup, 2)):
+                sm.sweep(sigma)
+            sm.nacc.zero_()
+            ms = timed(lambda: sm.sweep(sigma), nsw)
+            acc = float(sm.nacc.sum().item()) / (nw * d.nptot * (nsw + 1))
+            r[name] = {"sweep_ms": ms, "sweeps_per_s": 1e3 / ms,
+                       "electron_moves_per_s": 1e3 / ms * nw * d.nptot,
+                       "refresh_ms": timed(sm.refresh, 5), "acceptance": acc}
+            sm = None
+        r["eval_ms"] = timed(wk.start, 5)
+        delta = 0.05 * rs * rs / d.nptot
+        wk.start(delta=delta)
+        r["all_electron_step_ms"] = timed(lambda: wk.sweep(1, delta), 10)
+        res[tag] = r
+        wk = None
+        ctx.close()
+    n = res["n122"]
+    out = {
+        "metric": "single-electron sweeps/s (N=122, 4096 walkers, FP64 inverse)",
+        "value": n["fp64"]["sweeps_per_s"],
+        "unit": "sweeps/s",
+        "n_gpus": 1,
+        "steps": max(args.steps, 5),
+        "warmup": args.warmup,
+        "ms_per_step": n["fp64"]["sweep_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "single-electron moves with Sherman-Morrison updates, plain "
+                               "Slater-Jastrow N=122 (61+61) rs=10, Pade u2, zero backflow, "
+                               "200 k-shells, 4096 walkers, Gaussian step 0.6"},
         "workloads": res,
     }
     print(json.dumps(out))

@@ -381,6 +381,42 @@ int qmc_dmc_branch(qmc_ctx* ctx, int nw, double* d_wate, int64_t* d_rng2, int32_
                    int32_t* d_counts, void* stream);
 
 /*
+ * Single-electron VMC moves with Sherman-Morrison updates of the inverse
+ * Slater matrices.  No counterpart in the reference, whose move displaces
+ * every electron at once (move, reptation.f:4617-4680); for trial functions
+ * without backflow and three-body Jastrow (every backflow table zero, no
+ * u3; other decks return QMC_ERR_ARG), where moving one electron changes
+ * one column of its spin's matrix A(k, j) = exp(i g_k . r_j).
+ *
+ * qmc_sem_state: device buffers owned by the caller, for w->nw walkers.
+ * ainv holds per walker and spin the inverse, row = particle, column =
+ * orbital, mm x mm with mm the larger spin's particle count: complex
+ * double (re, im) with QMC_PREC_FP64, complex float with QMC_PREC_MIXED.
+ */
+typedef struct qmc_sem_state {
+  void* ainv;       /* [nw][2][mm][mm] complex                              */
+  double* dlogpsi;  /* [nw] added to: log|psi'| - log|psi| of each accepted
+                       move (log|R| - dU)                                   */
+  double* nacc;     /* [nw] added to: accepted moves                        */
+} qmc_sem_state;
+
+/* The inverses from the current positions w->x (FP64 Gauss-Jordan with
+ * partial pivoting), stored at the precision asked for. */
+int qmc_sem_refresh(qmc_ctx* ctx, const qmc_walkers* w, const qmc_sem_state* d, int precision,
+                    void* stream);
+
+/*
+ * Moves of electrons j0..j1-1 (0-based), one after the other, for every
+ * walker: a Gaussian step of width sigma per dimension (Box-Muller from the
+ * walker's rannyu stream w->rng: two draws for the step, a third for the
+ * acceptance), accepted with probability |R|^2 exp(-2 dU), where R is the
+ * determinant ratio (A^-1 a')_j and dU the change of the u2 sum.  Positions
+ * and inverses are updated in place; w->props, w->g, w->h are not.
+ */
+int qmc_sem_moves(qmc_ctx* ctx, const qmc_walkers* w, const qmc_sem_state* d, int j0, int j1,
+                  double sigma, int precision, void* stream);
+
+/*
  * Per-kernel timing with HIP events recorded on the launch stream around each
  * kernel of every evaluation while enabled (no counterpart in the reference,
  * which has no timers).  qmc_timing(ctx, 1) resets and enables,

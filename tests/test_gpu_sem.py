@@ -1,0 +1,263 @@
+"""Single-electron moves with Sherman-Morrison inverse updates
+(qmc_sem_refresh / qmc_sem_moves).  The reference has no such move, so the
+checks are what the mathematics fixes: the refreshed inverse against
+numpy's inverse of the orbital matrix; the inverse carried through a sweep
+of rank-one updates against a fresh inverse of the final positions; the
+summed log|psi| change of the accepted moves against the difference of two
+full evaluations (-ltf, the evaluation the reference's compute_properties
+performs); and the VMC energy of the single-electron walk against the
+all-electron walk of the reference's move on the same trial function."""
+import numpy as np
+import pytest
+
+from conftest import load_deck, pkg
+
+pytestmark = pytest.mark.gpu
+
+
+def _states(nw, base=3099):
+    abi = pkg("abi")
+    return [abi.limbs_to_state([3535, 3467, 3262, (base + 2 * w) % 4096]) for w in range(nw)]
+
+
+def _orbital_matrix(gvec, xs):
+    # A(k, j) = exp(i g_k . r_j) for the spin's m particles
+    m = xs.shape[0]
+    return np.exp(1j * (gvec[:m] @ xs.T))
+
+
+def _inverses(sm):
+    a = sm.ainv.cpu().numpy().astype(np.float64)
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def _setup(name, nw, precision=0, deck=None):
+    hip = pkg("hip")
+    d = deck if deck is not None else load_deck(name)
+    ctx = hip.Context(d, twist=(1, 1))
+    x = pkg("deckgen").lattice_configs(nw, d.nptot, d.el[0], 7)
+    wk = hip.Walkers(ctx, x, _states(nw))
+    wk.start()
+    sm = hip.SingleMoves(wk, precision=precision)
+    return d, ctx, wk, sm
+
+
+def _spins(d):
+    b = [0] + [int(sum(d.np[:t + 1])) for t in range(d.ntypes)]
+    return [(b[t], b[t + 1]) for t in range(d.ntypes) if b[t + 1] > b[t]]
+
+
+def _check_inverse_exact(d, ctx, wk, sm, tol):
+    import torch
+    torch.cuda.synchronize()
+    x = wk.x.cpu().numpy()
+    inv = _inverses(sm)
+    worst = 0.0
+    for w in range(x.shape[0]):
+        for sp, (a, b) in enumerate(_spins(d)):
+            m = b - a
+            A = _orbital_matrix(ctx.gvec, x[w, a:b])
+            ref = np.linalg.inv(A)
+            worst = max(worst, float(np.max(np.abs(inv[w, sp, :m, :m] - ref)) / np.max(np.abs(ref))))
+    assert worst < tol, worst
+    return worst
+
+
+@pytest.mark.parametrize("name", ["c26", "free26"])
+def test_refresh_inverse_matches_numpy(name):
+    print(f"\n[gpu] sem refresh {name}", flush=True)
+    d, ctx, wk, sm = _setup(name, 16)
+    sm.refresh()
+    worst = _check_inverse_exact(d, ctx, wk, sm, 1e-11)
+    print(f"   max rel error of the inverse {worst:.2e}", flush=True)
+
+
+def test_sweeps_keep_inverse_and_logpsi_exact_c26():
+    """Five sweeps of single-electron moves: the rank-one-updated inverse
+    equals a fresh inverse of the final positions, and the summed
+    log|R| - dU of the accepted moves equals -(ltf_after - ltf_before) of
+    full evaluations."""
+    print("\n[gpu] sem sweeps c26: inverse and log|psi| against full evaluations", flush=True)
+    import torch
+    d, ctx, wk, sm = _setup("c26", 64)
+    jltf = ctx.info["jltf"] - 1
+    ltf0 = wk.props[:, jltf].clone()
+    sm.refresh()
+    for _ in range(5):
+        sm.sweep(0.5)
+    worst = _check_inverse_exact(d, ctx, wk, sm, 1e-9)
+    wk.start()
+    torch.cuda.synchronize()
+    dl = sm.dlogpsi.cpu().numpy()
+    ref = -(wk.props[:, jltf] - ltf0).cpu().numpy()
+    acc = sm.nacc.cpu().numpy() / (5 * d.nptot)
+    err = float(np.max(np.abs(dl - ref)))
+    print(f"   inverse rel err {worst:.2e}  |dlogpsi - full| {err:.2e}  acceptance {acc.mean():.3f}", flush=True)
+    assert 0.2 < acc.mean() < 0.9
+    assert err < 1e-8
+
+
+def test_moves_range_and_deck_checks():
+    """Electron ranges out of bounds and decks with backflow or a
+    three-body Jastrow are refused."""
+    print("\n[gpu] sem argument checks", flush=True)
+    hip = pkg("hip")
+    d, ctx, wk, sm = _setup("c26", 4)
+    with pytest.raises(hip.QmcError):
+        sm.moves(0, d.nptot + 1, 0.5)
+    with pytest.raises(hip.QmcError):
+        sm.moves(3, 2, 0.5)
+    with pytest.raises(hip.QmcError):
+        sm.moves(0, 4, 0.0)
+    _, _, wk2, sm2 = _setup("n26", 4)
+    with pytest.raises(hip.QmcError):
+        sm2.refresh()
+
+
+def test_mixed_inverse_drift_and_logpsi():
+    """QMC_PREC_MIXED keeps the inverse in FP32: after one refresh its
+    distance from the FP64 inverse is FP32 rounding; after sweeps of FP32
+    rank-one updates it drifts, and a refresh brings it back.  The summed
+    log|psi| change stays within 1e-3 of the full evaluations over two
+    sweeps."""
+    print("\n[gpu] sem mixed precision c26", flush=True)
+    import torch
+    d, ctx, wk, sm = _setup("c26", 64, precision=1)
+    jltf = ctx.info["jltf"] - 1
+    ltf0 = wk.props[:, jltf].clone()
+    sm.refresh()
+    e0 = _check_inverse_exact(d, ctx, wk, sm, 1e-5)
+    for _ in range(2):
+        sm.sweep(0.5)
+    e2 = _check_inverse_exact(d, ctx, wk, sm, 1e-3)
+    wk.start()
+    torch.cuda.synchronize()
+    err = float(np.max(np.abs(sm.dlogpsi.cpu().numpy() + (wk.props[:, jltf] - ltf0).cpu().numpy())))
+    for _ in range(18):
+        sm.sweep(0.5)
+    e20 = _check_inverse_exact(d, ctx, wk, sm, 1e-2)
+    sm.refresh()
+    er = _check_inverse_exact(d, ctx, wk, sm, 1e-5)
+    print(f"   inverse rel err: refresh {e0:.1e}, 2 sweeps {e2:.1e}, 20 sweeps {e20:.1e}, "
+          f"refreshed {er:.1e}; |dlogpsi - full| after 2 sweeps {err:.1e}", flush=True)
+    assert err < 1e-3
+
+
+def _hard_core_configs(nw, n, L, seed, rmin=0.6):
+    """Uniform positions with no two particles closer than rmin (minimum
+    image), placed one at a time by rejection."""
+    rng = np.random.default_rng(seed)
+    x = np.empty((nw, n, 2))
+    for w in range(nw):
+        k = 0
+        while k < n:
+            p = rng.uniform(-L / 2, L / 2, 2)
+            dd = x[w, :k] - p
+            dd -= L * np.round(dd / L)
+            if k == 0 or np.min(np.hypot(dd[:, 0], dd[:, 1])) >= rmin:
+                x[w, k] = p
+                k += 1
+    return x
+
+
+def _energy_all_electron(ctx, d, nw, delta, neq, nmeas):
+    """Mean energy per step of the all-electron walk over the walkers that
+    move.  Without a drift cutoff, a walker that comes close to a node of
+    the determinant gets a drift so large that the drift-diffusion proposal
+    is no longer accepted, and stays for thousands of steps (at N=26 and
+    delta 0.25, some 8% of walkers within 2000 steps); the deck is therefore
+    run with the reference's drift cutoff (adrift 0.5), which bounds the
+    drift, and the test asks that every walker moves."""
+    import torch
+    hip = pkg("hip")
+    x = _hard_core_configs(nw, d.nptot, d.el[0], 11)
+    wk = hip.Walkers(ctx, x, _states(nw, 1001))
+    wk.start(delta=delta)
+    wk.sweep(neq, delta)
+    je = ctx.info["jetot"] - 1
+    es = []
+    for _ in range(nmeas):
+        wk.sweep(1, delta)
+        es.append(wk.props[:, je].clone())
+    torch.cuda.synchronize()
+    moving = (wk.age < 50).cpu().numpy()
+    e = torch.stack(es).cpu().numpy()[:, moving]
+    return e.mean(1), int(moving.sum())
+
+
+def _energy_single(ctx, d, nw, sigma, neq, nmeas, precision=0, refresh_every=10):
+    import torch
+    hip = pkg("hip")
+    x = pkg("deckgen").lattice_configs(nw, d.nptot, d.el[0], 13)
+    wk = hip.Walkers(ctx, x, _states(nw, 2001))
+    wk.start()
+    sm = hip.SingleMoves(wk, precision=precision)
+    sm.refresh()
+    je = ctx.info["jetot"] - 1
+    es = []
+    for it in range(neq + nmeas):
+        sm.sweep(sigma)
+        if (it + 1) % refresh_every == 0:
+            sm.refresh()
+        if it >= neq:
+            wk.start()
+            es.append(wk.props[:, je].mean())
+    torch.cuda.synchronize()
+    acc = float(sm.nacc.sum().item()) / (nw * d.nptot * (neq + nmeas))
+    return torch.stack(es).cpu().numpy(), acc
+
+
+def _blocked(e, nb=20):
+    b = np.asarray(e[: len(e) // nb * nb]).reshape(nb, -1).mean(1)
+    return float(b.mean()), float(b.std(ddof=1) / np.sqrt(nb))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_energy_agrees_with_all_electron_walk_c26(precision):
+    """Statistical agreement of the VMC energy: the single-electron walk
+    (FP64, or with the FP32 inverse refreshed every 10 sweeps) and the
+    all-electron drift-diffusion walk (with the drift cutoff adrift 0.5,
+    which the single-electron walk does not read) sample the same
+    |psi|^2."""
+    print(f"\n[gpu] sem energy vs all-electron walk c26 precision {precision}", flush=True)
+    d = load_deck("c26")
+    d.adrift = 0.5
+    hip = pkg("hip")
+    ctx = hip.Context(d, twist=(1, 1))
+    rs2 = 1.0 / d.hbs2m[0]
+    ea, nmove = _energy_all_electron(ctx, d, 512, 0.01 * rs2, 400, 2000)
+    es, acc = _energy_single(ctx, d, 512, 0.5, 40, 200, precision=precision)
+    ma, sa = _blocked(ea)
+    ms, ss = _blocked(es)
+    z = abs(ma - ms) / np.hypot(sa, ss)
+    print(f"   all-electron {ma:.6f} +- {sa:.6f} ({nmove} of 512 walkers move)   single-electron "
+          f"{ms:.6f} +- {ss:.6f} (acc {acc:.3f})  z {z:.2f}", flush=True)
+    assert nmove == 512
+    assert z < 4.5
+    assert np.hypot(sa, ss) < 2e-3 * abs(ma)
+
+
+def test_sweeps_n122_plain_slater_jastrow(tmp_path):
+    """At N=122 (61 per spin, the largest per-spin matrix of the benchmark
+    sizes): a generated deck with a Pade two-body Jastrow and no backflow;
+    the updated inverse after three sweeps equals a fresh inverse, and the
+    summed log|psi| change equals the full evaluations'."""
+    print("\n[gpu] sem sweeps N=122 (generated plain Slater-Jastrow deck)", flush=True)
+    import torch
+    dg = pkg("deckgen")
+    dg.make_deck(str(tmp_path), "p122", 61, 61, 5.0, 5, nshell_k=12, u3=False, backflow="zero",
+                 jastrow="pade")
+    d = pkg("deck").read_deck(str(tmp_path))
+    d, ctx, wk, sm = _setup(None, 16, deck=d)
+    jltf = ctx.info["jltf"] - 1
+    ltf0 = wk.props[:, jltf].clone()
+    sm.refresh()
+    for _ in range(3):
+        sm.sweep(0.5)
+    worst = _check_inverse_exact(d, ctx, wk, sm, 1e-8)
+    wk.start()
+    torch.cuda.synchronize()
+    err = float(np.max(np.abs(sm.dlogpsi.cpu().numpy() + (wk.props[:, jltf] - ltf0).cpu().numpy())))
+    acc = float(sm.nacc.sum().item()) / (16 * 3 * d.nptot)
+    print(f"   inverse rel err {worst:.2e}  |dlogpsi - full| {err:.2e}  acceptance {acc:.3f}", flush=True)
+    assert err < 1e-7
