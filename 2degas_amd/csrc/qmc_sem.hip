@@ -20,6 +20,60 @@
 // recomputed in FP64 by qmc_sem_refresh (Gauss-Jordan with partial pivoting).
 #include "qmc_internal.h"
 
+// exp and log for the move loop, their constants kept in scalar registers
+// (qk<true>) so that the loop does not hold them in vector registers next to
+// the inverse.  exp: Cody-Waite reduction x = n ln2 + r (ln2 over two
+// doubles), |r| <= ln2 / 2, the Taylor series of e^r through r^13
+// (truncation below 5e-18), scaled by 2^n; arguments are clamped to
+// [-746, 710], beyond which the result is 0 or +inf.  log: x = m 2^e with
+// m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m - 1) / (m + 1),
+// |s| <= 0.1716, the series through s^21 (truncation below 1e-18), plus
+// e ln2; x > 0.
+__device__ __forceinline__ double sem_exp(double x) {
+  x = fmin(fmax(x, -746.0), 710.0);
+  const double n = rint(x * qk<true>(0x1.71547652b82fep+0));
+  double r = fma(-n, qk<true>(0x1.62e42fefa39efp-1), x);
+  r = fma(-n, qk<true>(0x1.abc9e3b39803fp-56), r);
+  double p = qk<true>(1.0 / 6227020800.0);
+  p = fma(p, r, qk<true>(1.0 / 479001600.0));
+  p = fma(p, r, qk<true>(1.0 / 39916800.0));
+  p = fma(p, r, qk<true>(1.0 / 3628800.0));
+  p = fma(p, r, qk<true>(1.0 / 362880.0));
+  p = fma(p, r, qk<true>(1.0 / 40320.0));
+  p = fma(p, r, qk<true>(1.0 / 5040.0));
+  p = fma(p, r, qk<true>(1.0 / 720.0));
+  p = fma(p, r, qk<true>(1.0 / 120.0));
+  p = fma(p, r, qk<true>(1.0 / 24.0));
+  p = fma(p, r, qk<true>(1.0 / 6.0));
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, (int)n);
+}
+
+__device__ __forceinline__ double sem_log(double x) {
+  int e;
+  double m = frexp(x, &e);
+  if (m < qk<true>(0x1.6a09e667f3bcdp-1)) {
+    m *= 2.0;
+    e -= 1;
+  }
+  const double sv = (m - 1.0) / (m + 1.0), s2 = sv * sv;
+  double p = qk<true>(1.0 / 21.0);
+  p = fma(p, s2, qk<true>(1.0 / 19.0));
+  p = fma(p, s2, qk<true>(1.0 / 17.0));
+  p = fma(p, s2, qk<true>(1.0 / 15.0));
+  p = fma(p, s2, qk<true>(1.0 / 13.0));
+  p = fma(p, s2, qk<true>(1.0 / 11.0));
+  p = fma(p, s2, qk<true>(1.0 / 9.0));
+  p = fma(p, s2, qk<true>(1.0 / 7.0));
+  p = fma(p, s2, qk<true>(1.0 / 5.0));
+  p = fma(p, s2, qk<true>(1.0 / 3.0));
+  const double lm = 2.0 * fma(sv * s2, p, sv);
+  const double de = (double)e;
+  return fma(de, qk<true>(0x1.62e42fefa39efp-1), fma(de, qk<true>(0x1.abc9e3b39803fp-56), lm));
+}
+
 template <typename Z>
 struct SemZ;
 template <>
@@ -134,73 +188,115 @@ __global__ __launch_bounds__(256) void k_sem_inverse(DevSys s, int nw, const dou
   }
 }
 
-// Moves of electrons j0..j1-1, in order: one 256-thread workgroup per
-// walker, holding in LDS the inverses of the spins the range touches (sp0
-// and up) and the positions for the call.  Per move: every thread draws the
-// same three uniforms from the walker's stream (so no broadcast is needed);
-// threads 0..m-1 form the new orbital column a'; four threads per row l
-// form t_l = (A^-1 a')_l while the same threads sum the Jastrow change over
-// the partners; after a barrier every thread holds R = t_j' and dU and
-// takes the same decision, and an accepted move updates the m x m inverse,
-// 256 threads over its elements.  Two barriers per move, reached by every
-// thread.
-template <typename Z>
-__global__ __launch_bounds__(256) void k_sem_moves(DevSys s, int j0, int j1, int sp0, double sigma,
-                                                   double* __restrict__ x, int64_t* __restrict__ rng,
-                                                   Z* __restrict__ ainv, int mm, double* __restrict__ dlogpsi,
-                                                   double* __restrict__ nacc) {
+// Moves of electrons j0..j1-1 of spin sp, in order: one workgroup of
+// MR x CG threads per walker, the walker's m x m inverse (m <= MR) held in
+// registers: thread (l, c) = (tid / CG, tid % CG) keeps row l at columns
+// c, c + CG, ..., MR / CG complex entries.  The proposals come first: lane i
+// of the first wavefront jumps the walker's stream 3i draws ahead and forms
+// move i's Gaussian step (Box-Muller) and acceptance uniform, kept in LDS.
+// Per move, threads 0..m-1 put the new orbital column a' in LDS; each thread
+// forms its part of t_l = (A^-1 a')_l and the CG threads of a row sum them
+// by lane shuffles; the threads of row j' put that row and R = t_j' in LDS
+// while every thread sums its share of the Jastrow change; after the
+// barrier every thread takes the same decision, and an accepted move
+// updates each thread's entries from its own t_l and the shared row.
+// Entries beyond m are zero and stay zero.  Two barriers per move, reached
+// by every thread.  The log|R| of the accepted moves are summed as a
+// product kept as mantissa and exponent (frexp), one log at the end.
+// The FP64 inverse of a 64-row spin takes 64 registers per thread; left to
+// itself the compiler spends some 110 more and fits 2 waves per SIMD.  A
+// bound of 3 costs a few spilled registers and is 17% faster at N=122
+// (measured, 4096 walkers).  Other shapes fit 4 or more waves unbounded.
+template <typename Z, int MR>
+struct SemWaves {
+  static constexpr int value = (sizeof(Z) == 16 && MR == 64) ? 3 : 1;
+};
+
+template <typename Z, int MR, int CG>
+__global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves(DevSys s, int j0, int j1, int sp, double sigma,
+                                                      double* __restrict__ x, int64_t* __restrict__ rng,
+                                                      Z* __restrict__ ainv, int mm, double* __restrict__ dlogpsi,
+                                                      double* __restrict__ nacc) {
   typedef typename SemZ<Z>::R R;
+  constexpr int T = MR * CG, EPT = MR / CG, NWV = T / 64 > 0 ? T / 64 : 1;
+  __shared__ Z ap[MR];    // new orbital column a'
+  __shared__ Z rowj[MR];  // row j' of the inverse before the update
+  __shared__ Z rz_sh;     // R
+  __shared__ double red[NWV];
+  __shared__ double2 step[64];  // move i's displacement
+  __shared__ double uacc[64];   // move i's acceptance uniform
   extern __shared__ __attribute__((aligned(16))) unsigned char sem_lds[];
+  double* xs = reinterpret_cast<double*>(sem_lds);  // positions [N][2]
   const int w = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int N = s.nptot;
-  const int nsp = ptype(s, j1 - 1) - sp0 + 1;
-  Z* B = reinterpret_cast<Z*>(sem_lds);  // [nsp][mm][mm]
-  Z* ap = B + nsp * mm * mm;             // new orbital column a'
-  Z* rowj = ap + 64;                      // row j' of the inverse before the update
-  Z* tv = rowj + 64;                      // t = A^-1 a'
-  double* xs = reinterpret_cast<double*>(tv + 64);  // positions [N][2]
-  double* red = xs + 2 * N;                         // per-wavefront Jastrow sums
-  const Z* gin = ainv + ((int64_t)w * 2 + sp0) * mm * mm;
-  for (int e = tid; e < nsp * mm * mm; e += 256) B[e] = gin[e];
-  for (int e = tid; e < 2 * N; e += 256) xs[e] = x[(int64_t)w * 2 * N + e];
-  uint64_t st = (uint64_t)rng[w];
-  double dl = 0.0, na = 0.0;
-  __syncthreads();
-  for (int j = j0; j < j1; j++) {
-    const int sp = ptype(s, j);
-    const int m = pick2(s.np, sp), jj = j - pick2(s.ipf, sp);
-    Z* A = B + (sp - sp0) * mm * mm;
-    // the proposal: a Gaussian step of width sigma per dimension (Box-Muller)
+  const int N = s.nptot, nmv = j1 - j0;
+  const int m = pick2(s.np, sp), p0 = pick2(s.ipf, sp);
+  const int l = tid / CG, cg = tid % CG;
+  const Z* gin = ainv + ((int64_t)w * 2 + sp) * mm * mm;
+  Z b[EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int k = cg + CG * i;
+    b[i] = (l < m && k < m) ? gin[l * mm + k] : Z{(R)0, (R)0};
+  }
+  if (tid < MR) ap[tid] = Z{(R)0, (R)0};
+  for (int e = tid; e < 2 * N; e += T) xs[e] = x[(int64_t)w * 2 * N + e];
+  const uint64_t st0 = (uint64_t)rng[w];
+  if (tid < nmv) {
+    // the proposal of move tid: a Gaussian step of width sigma per dimension
+    uint64_t st = (st0 * lcg_pow(3 * (uint64_t)tid)) & ((1ULL << 48) - 1ULL);
     const double u1 = rannyu(st), u2 = rannyu(st), ua = rannyu(st);
     double sv, cv;
     qsincos(6.283185307179586 * u2, &sv, &cv);
-    const double rad = sigma * sqrt(-2.0 * log(u1));
+    const double rad = sigma * sqrt(-2.0 * sem_log(u1));
+    step[tid] = make_double2(rad * cv, rad * sv);
+    uacc[tid] = ua;
+  }
+  double du_acc = 0.0, na = 0.0, pm = 1.0;
+  int pe = 0;
+  __syncthreads();
+  for (int j = j0; j < j1; j++) {
+    const int jj = j - p0;
+    const double2 dx = step[j - j0];
     const double xo = xs[2 * j], yo = xs[2 * j + 1];
-    const double xn = xo + rad * cv, yn = yo + rad * sv;
+    const double xn = xo + dx.x, yn = yo + dx.y;
     if (tid < m) {
       double so, co;
-      qsincos(s.gvec[2 * tid] * xn + s.gvec[2 * tid + 1] * yn, &so, &co);
+      qsincos_sreg(s.gvec[2 * tid] * xn + s.gvec[2 * tid + 1] * yn, &so, &co);
       ap[tid] = Z{(R)co, (R)so};
     }
     __syncthreads();
-    // t_l over four threads per row
-    const int l = tid >> 2, q = tid & 3;
-    R tr = 0, ti = 0;
-    if (l < m)
-      for (int k = q; k < m; k += 4) {
-        const Z a = A[l * mm + k], b = ap[k];
-        tr += a.x * b.x - a.y * b.y;
-        ti += a.x * b.y + a.y * b.x;
+    // t as explicit FMAs into two accumulator pairs (even and odd entries)
+    R tr = 0, ti = 0, tr2 = 0, ti2 = 0;
+#pragma unroll
+    for (int i = 0; i < EPT; i += 2) {
+      const Z a = ap[cg + CG * i];
+      tr = fma(b[i].x, a.x, tr);
+      tr = fma(-b[i].y, a.y, tr);
+      ti = fma(b[i].x, a.y, ti);
+      ti = fma(b[i].y, a.x, ti);
+      if (i + 1 < EPT) {
+        const Z a2 = ap[cg + CG * (i + 1)];
+        tr2 = fma(b[i + 1].x, a2.x, tr2);
+        tr2 = fma(-b[i + 1].y, a2.y, tr2);
+        ti2 = fma(b[i + 1].x, a2.y, ti2);
+        ti2 = fma(b[i + 1].y, a2.x, ti2);
       }
-    tr += __shfl_xor(tr, 1, 64);
-    ti += __shfl_xor(ti, 1, 64);
-    tr += __shfl_xor(tr, 2, 64);
-    ti += __shfl_xor(ti, 2, 64);
-    if (q == 0 && l < m) tv[l] = Z{tr, ti};
-    if (tid < m) rowj[tid] = A[jj * mm + tid];
+    }
+    tr += tr2;
+    ti += ti2;
+#pragma unroll
+    for (int o = 1; o < CG; o <<= 1) {
+      tr += __shfl_xor(tr, o, 64);
+      ti += __shfl_xor(ti, o, 64);
+    }
+    if (l == jj) {
+#pragma unroll
+      for (int i = 0; i < EPT; i++) rowj[cg + CG * i] = b[i];
+      if (cg == 0) rz_sh = Z{tr, ti};
+    }
     // the Jastrow change over the partners of j
     double du = 0.0;
-    for (int p = tid; p < N; p += 256) {
+    for (int p = tid; p < N; p += T) {
       if (p == j) continue;
       const TabRef tu2 = tabref<TK_U2>(s, tpair(sp, ptype(s, p)));
       if (tu2.row0 < 0) continue;
@@ -214,39 +310,69 @@ __global__ __launch_bounds__(256) void k_sem_moves(DevSys s, int j0, int j1, int
     du = wave_sum(du);
     if (lane == 0) red[wv] = du;
     __syncthreads();
-    du = red[0] + red[1] + red[2] + red[3];
-    const Z rz = tv[jj];
+    du = 0.0;
+#pragma unroll
+    for (int k = 0; k < NWV; k++) du += red[k];
+    const Z rz = rz_sh;
     const double mod2 = (double)rz.x * (double)rz.x + (double)rz.y * (double)rz.y;
-    if (ua < mod2 * exp(-2.0 * du)) {
+    if (uacc[j - j0] < mod2 * sem_exp(-2.0 * du)) {
       // A^-1 -= (t - e_j') (row j' of A^-1) / R, 1/R as conj(R) / |R|^2
       const R inv2 = (R)(1.0 / mod2);
-      for (int e = tid; e < m * m; e += 256) {
-        const int r = e / m, k = e - r * m;
-        const Z t = tv[r], b = rowj[k];
-        const R vr = r == jj ? t.x - (R)1 : t.x, vi = t.y;
-        const R fr = -(vr * rz.x + vi * rz.y) * inv2, fi = -(vi * rz.x - vr * rz.y) * inv2;
-        Z a = A[r * mm + k];
-        a.x += fr * b.x - fi * b.y;
-        a.y += fr * b.y + fi * b.x;
-        A[r * mm + k] = a;
+      const R vr = l == jj ? tr - (R)1 : tr;
+      const R fr = -(vr * rz.x + ti * rz.y) * inv2, fi = -(ti * rz.x - vr * rz.y) * inv2;
+#pragma unroll
+      for (int i = 0; i < EPT; i++) {
+        const Z q = rowj[cg + CG * i];
+        b[i].x = fma(fr, q.x, b[i].x);
+        b[i].x = fma(-fi, q.y, b[i].x);
+        b[i].y = fma(fr, q.y, b[i].y);
+        b[i].y = fma(fi, q.x, b[i].y);
       }
       if (tid == 0) {
         xs[2 * j] = xn;
         xs[2 * j + 1] = yn;
       }
-      dl += 0.5 * log(mod2) - du;
+      int e;
+      pm = frexp(pm * mod2, &e);
+      pe += e;
+      du_acc += du;
       na += 1.0;
     }
   }
   __syncthreads();
-  Z* gout = ainv + ((int64_t)w * 2 + sp0) * mm * mm;
-  for (int e = tid; e < nsp * mm * mm; e += 256) gout[e] = B[e];
-  for (int e = tid; e < 2 * N; e += 256) x[(int64_t)w * 2 * N + e] = xs[e];
+  Z* gout = ainv + ((int64_t)w * 2 + sp) * mm * mm;
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int k = cg + CG * i;
+    if (l < m && k < m) gout[l * mm + k] = b[i];
+  }
+  for (int e = tid; e < 2 * N; e += T) x[(int64_t)w * 2 * N + e] = xs[e];
   if (tid == 0) {
-    rng[w] = (int64_t)st;
-    dlogpsi[w] += dl;
+    rng[w] = (int64_t)((st0 * lcg_pow(3 * (uint64_t)nmv)) & ((1ULL << 48) - 1ULL));
+    dlogpsi[w] += 0.5 * (sem_log(pm) + pe * 0.6931471805599453) - du_acc;
     nacc[w] += na;
   }
+}
+
+template <typename Z, int MR>
+static int launch_sem_moves(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, int j0, int j1, int sp,
+                            double sigma, int mm, hipStream_t st) {
+  constexpr int CG = 4;
+  const size_t lds = 2 * (size_t)c->s.nptot * sizeof(double);
+  if (lds > 65536) return arg_fail("too many particles for the single-electron move kernel", QMC_ERR_LIMIT);
+  hipLaunchKernelGGL((k_sem_moves<Z, MR, CG>), dim3(w->nw), dim3(MR * CG), lds, st, c->s, j0, j1, sp, sigma,
+                     w->x, w->rng, (Z*)d->ainv, mm, d->dlogpsi, d->nacc);
+  HIPCHK(hipGetLastError());
+  return QMC_OK;
+}
+
+template <typename Z>
+static int sem_moves_spin(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, int j0, int j1, int sp,
+                          double sigma, int mm, hipStream_t st) {
+  const int m = c->s.np[sp];
+  if (m <= 16) return launch_sem_moves<Z, 16>(c, w, d, j0, j1, sp, sigma, mm, st);
+  if (m <= 32) return launch_sem_moves<Z, 32>(c, w, d, j0, j1, sp, sigma, mm, st);
+  return launch_sem_moves<Z, 64>(c, w, d, j0, j1, sp, sigma, mm, st);
 }
 
 static int sem_check(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, int precision) {
@@ -295,22 +421,14 @@ extern "C" int qmc_sem_moves(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_sta
   if (!(sigma > 0.0)) return arg_fail("sigma must be positive");
   if (w->nw == 0 || j0 == j1) return QMC_OK;
   HIPCHK(hipSetDevice(c->device));
-  const int mm = sem_mm(c), N = c->s.nptot;
-  const int sp0 = (c->s.ntypes > 1 && j0 >= c->s.ipf[1]) ? 1 : 0;
-  const int sp1 = (c->s.ntypes > 1 && j1 - 1 >= c->s.ipf[1]) ? 1 : 0;
-  const size_t zs = precision == QMC_PREC_MIXED ? sizeof(float2) : sizeof(double2);
-  const size_t lds = (size_t)(sp1 - sp0 + 1) * mm * mm * zs + 3 * 64 * zs + (2 * (size_t)N + 4) * sizeof(double);
-  int lds_max = 0;
-  HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-  if (lds > (size_t)lds_max) return arg_fail("a walker's inverses do not fit in LDS: move one spin per call", QMC_ERR_LIMIT);
-  const void* kf = precision == QMC_PREC_MIXED ? (const void*)k_sem_moves<float2> : (const void*)k_sem_moves<double2>;
-  if (lds > 65536) HIPCHK(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  if (precision == QMC_PREC_MIXED)
-    hipLaunchKernelGGL(k_sem_moves<float2>, dim3(w->nw), dim3(256), lds, (hipStream_t)stream, c->s, j0, j1, sp0,
-                       sigma, w->x, w->rng, (float2*)d->ainv, mm, d->dlogpsi, d->nacc);
-  else
-    hipLaunchKernelGGL(k_sem_moves<double2>, dim3(w->nw), dim3(256), lds, (hipStream_t)stream, c->s, j0, j1, sp0,
-                       sigma, w->x, w->rng, (double2*)d->ainv, mm, d->dlogpsi, d->nacc);
-  HIPCHK(hipGetLastError());
+  const int mm = sem_mm(c);
+  // one launch per spin the range touches, in order
+  for (int sp = 0; sp < c->s.ntypes; sp++) {
+    const int a = std::max(j0, c->s.ipf[sp]), b = std::min(j1, c->s.ipf[sp] + c->s.np[sp]);
+    if (a >= b) continue;
+    rc = precision == QMC_PREC_MIXED ? sem_moves_spin<float2>(c, w, d, a, b, sp, sigma, mm, (hipStream_t)stream)
+                                     : sem_moves_spin<double2>(c, w, d, a, b, sp, sigma, mm, (hipStream_t)stream);
+    if (rc) return rc;
+  }
   return QMC_OK;
 }

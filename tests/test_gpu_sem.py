@@ -97,6 +97,30 @@ def test_sweeps_keep_inverse_and_logpsi_exact_c26():
     assert err < 1e-8
 
 
+def test_split_ranges_replay_the_same_walk():
+    """The moves of a range draw three uniforms each from the walker's
+    stream in order, so moving electrons 0..4, then 5..12, then 13..25
+    gives bit for bit the walk of one call over 0..25 (the same streams,
+    positions, inverses and accepted counts; the log|psi| sums, grouped per
+    call, agree to rounding)."""
+    print("\n[gpu] sem split ranges c26", flush=True)
+    import torch
+    runs = []
+    for split in ([0, 26], [0, 5, 13, 26]):
+        d, ctx, wk, sm = _setup("c26", 32)
+        sm.refresh()
+        for _ in range(2):
+            for a, b in zip(split[:-1], split[1:]):
+                sm.moves(a, b, 0.5)
+        torch.cuda.synchronize()
+        runs.append([t.cpu().numpy().copy() for t in (wk.x, wk.rng, sm.ainv, sm.dlogpsi, sm.nacc)])
+    for k, (u, v) in enumerate(zip(*runs)):
+        if k == 3:
+            assert np.max(np.abs(u - v)) < 1e-12
+        else:
+            assert np.array_equal(u, v), k
+
+
 def test_moves_range_and_deck_checks():
     """Electron ranges out of bounds and decks with backflow or a
     three-body Jastrow are refused."""
