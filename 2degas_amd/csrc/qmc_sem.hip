@@ -85,107 +85,126 @@ struct SemZ<float2> {
   typedef float R;
 };
 
-// The inverse of A for (walker, spin): A built from the positions in LDS
-// and inverted in place by Gauss-Jordan with partial pivoting (largest
-// |re| + |im| in the column), the column order restored at the end.
-template <typename Z>
-__global__ __launch_bounds__(256) void k_sem_inverse(DevSys s, int nw, const double* __restrict__ x,
-                                                     Z* __restrict__ ainv, int mm) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char sem_lds[];
-  double2* A = reinterpret_cast<double2*>(sem_lds);  // m x m, (r, c) at r * m + c
-  __shared__ int perm[64];
-  __shared__ double2 colc[64];
-  __shared__ int sh_piv;
-  const int w = blockIdx.x, sp = blockIdx.y, tid = threadIdx.x;
-  if (w >= nw) return;
-  const int m = pick2(s.np, sp);
-  if (m == 0) return;
-  const int p0 = pick2(s.ipf, sp);
-  const double* xw = x + (int64_t)w * 2 * s.nptot;
-  // A(k, j) = exp(i g_k . r_j)
-  for (int e = tid; e < m * m; e += 256) {
-    const int k = e / m, j = e % m;
-    double sv, cv;
-    qsincos(s.gvec[2 * k] * xw[2 * (p0 + j)] + s.gvec[2 * k + 1] * xw[2 * (p0 + j) + 1], &sv, &cv);
-    A[e] = make_double2(cv, sv);
+// The inverse of A for (walker, spin sp): one workgroup of MR x CG threads
+// per walker, A(l, k) = exp(i g_l . r_k) built in registers with the layout
+// of the move kernel (thread (l, c) keeps row l at columns c, c + CG, ...),
+// inverted by Gauss-Jordan exchange steps with partial pivoting, in FP64.
+// Step c pivots on the row p not yet used with the largest |re| + |im| in
+// column c (ties: the lowest row) and exchanges row p's output with column
+// c's input: T(p, c) <- 1 / piv, T(p, k) <- -T(p, k) / piv,
+// T(r, c) <- T(r, c) / piv, T(r, k) <- T(r, k) - T(r, c) T(p, k) / piv.
+// After the m steps, with p_c the pivot row of step c,
+// A^-1(c, p_c') = T(p_c, c'), which the threads store as they hold it.
+// Rows are never moved: the pivot search is a lane-shuffle argmax per
+// wavefront and one over the wavefronts in LDS, the pivot row goes through
+// LDS, and each thread updates its own entries; two barriers per step.
+template <typename Z, int MR, int CG>
+__global__ __launch_bounds__(MR* CG) void k_sem_inverse(DevSys s, int sp, const double* __restrict__ x,
+                                                        Z* __restrict__ ainv, int mm) {
+  typedef typename SemZ<Z>::R R;
+  constexpr int T = MR * CG, EPT = MR / CG, NWV = T / 64 > 0 ? T / 64 : 1;
+  __shared__ double2 prow[MR];  // pivot row of the step
+  __shared__ double cbest[NWV];
+  __shared__ int cidx[NWV];
+  __shared__ int pr[MR], rk[MR];  // pivot row of step c, step of pivot row l
+  const int w = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m = pick2(s.np, sp), p0 = pick2(s.ipf, sp);
+  const int l = tid / CG, cg = tid % CG;
+  const double* xw = x + ((int64_t)w * s.nptot + p0) * 2;
+  const double gx = l < m ? s.gvec[2 * l] : 0.0, gy = l < m ? s.gvec[2 * l + 1] : 0.0;
+  double2 b[EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int k = cg + CG * i;
+    b[i] = make_double2(0.0, 0.0);
+    if (l < m && k < m) {
+      double sv, cv;
+      qsincos(gx * xw[2 * k] + gy * xw[2 * k + 1], &sv, &cv);
+      b[i] = make_double2(cv, sv);
+    }
+  }
+  bool used = l >= m;
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    for (int c4 = 0; c4 < CG; c4++) {
+      const int c = CG * i + c4;
+      if (c >= m) break;
+      // T(l, c), held by the thread of row l with cg == c4
+      const int src = (lane & ~(CG - 1)) | c4;
+      const double2 f = make_double2(__shfl(b[i].x, src, 64), __shfl(b[i].y, src, 64));
+      double mag = used ? -1.0 : fabs(f.x) + fabs(f.y);
+      int idx = l;
+#pragma unroll
+      for (int o = CG; o < 64; o <<= 1) {
+        const double om = __shfl_xor(mag, o, 64);
+        const int oi = __shfl_xor(idx, o, 64);
+        if (om > mag || (om == mag && oi < idx)) {
+          mag = om;
+          idx = oi;
+        }
+      }
+      if (lane == 0) {
+        cbest[wv] = mag;
+        cidx[wv] = idx;
+      }
+      __syncthreads();
+      double bm = cbest[0];
+      int p = cidx[0];
+#pragma unroll
+      for (int k = 1; k < NWV; k++)
+        if (cbest[k] > bm || (cbest[k] == bm && cidx[k] < p)) {
+          bm = cbest[k];
+          p = cidx[k];
+        }
+      if (l == p) {
+#pragma unroll
+        for (int ii = 0; ii < EPT; ii++) prow[cg + CG * ii] = b[ii];
+        used = true;
+        if (cg == 0) rk[p] = c;
+      }
+      if (tid == 0) pr[c] = p;
+      __syncthreads();
+      const double2 pv = prow[c];
+      const double dd = 1.0 / (pv.x * pv.x + pv.y * pv.y);
+      const double2 ip = make_double2(pv.x * dd, -pv.y * dd);
+      const double2 fi = make_double2(f.x * ip.x - f.y * ip.y, f.x * ip.y + f.y * ip.x);
+#pragma unroll
+      for (int ii = 0; ii < EPT; ii++) {
+        const int k = cg + CG * ii;
+        const double2 q = prow[k];
+        const double2 u = make_double2(q.x * ip.x - q.y * ip.y, q.x * ip.y + q.y * ip.x);
+        if (l == p) b[ii] = k == c ? ip : make_double2(-u.x, -u.y);
+        else if (k == c) b[ii] = fi;
+        else b[ii] = make_double2(b[ii].x - (f.x * u.x - f.y * u.y), b[ii].y - (f.x * u.y + f.y * u.x));
+      }
+    }
   }
   __syncthreads();
-  for (int c = 0; c < m; c++) {
-    if (tid < 64) {
-      // pivot row: the largest |re| + |im| among rows c..m-1 of column c
-      double best = -1.0;
-      int bi = c;
-      for (int r = c + tid; r < m; r += 64) {
-        const double2 a = A[r * m + c];
-        const double v = fabs(a.x) + fabs(a.y);
-        if (v > best) {
-          best = v;
-          bi = r;
-        }
-      }
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ob = __shfl_xor(best, o, 64);
-        const int oi = __shfl_xor(bi, o, 64);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
-        }
-      }
-      if (tid == 0) {
-        sh_piv = bi;
-        perm[c] = bi;
-      }
+  if (l < m) {
+    Z* out = ainv + ((int64_t)w * 2 + sp) * mm * mm + (int64_t)rk[l] * mm;
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int k = cg + CG * i;
+      if (k < m) out[pr[k]] = Z{(R)b[i].x, (R)b[i].y};
     }
-    __syncthreads();
-    const int pr = sh_piv;
-    if (pr != c)
-      for (int j = tid; j < m; j += 256) {
-        const double2 t = A[c * m + j];
-        A[c * m + j] = A[pr * m + j];
-        A[pr * m + j] = t;
-      }
-    __syncthreads();
-    const double2 pv = A[c * m + c];
-    const double d = pv.x * pv.x + pv.y * pv.y;
-    const double2 inv = make_double2(pv.x / d, -pv.y / d);
-    __syncthreads();
-    // row c scaled by 1/pivot, its pivot entry replaced by 1/pivot
-    for (int j = tid; j < m; j += 256) {
-      const double2 a = j == c ? make_double2(1.0, 0.0) : A[c * m + j];
-      A[c * m + j] = make_double2(a.x * inv.x - a.y * inv.y, a.x * inv.y + a.y * inv.x);
-    }
-    __syncthreads();
-    // column c before the rows change it
-    if (tid < m) colc[tid] = A[tid * m + c];
-    __syncthreads();
-    // the other rows: row r -= A(r, c) row c, with A(r, c) -> -A(r, c) / pivot
-    for (int e = tid; e < m * m; e += 256) {
-      const int r = e / m, j = e % m;
-      if (r == c) continue;
-      const double2 f = colc[r];
-      const double2 u = A[c * m + j];
-      if (j == c) A[e] = make_double2(-(f.x * u.x - f.y * u.y), -(f.x * u.y + f.y * u.x));
-      else A[e] = make_double2(A[e].x - (f.x * u.x - f.y * u.y), A[e].y - (f.x * u.y + f.y * u.x));
-    }
-    __syncthreads();
   }
-  // the row swaps of A are column swaps of the inverse, undone last first
-  for (int c = m - 1; c >= 0; c--) {
-    const int pr = perm[c];
-    if (pr != c)
-      for (int r = tid; r < m; r += 256) {
-        const double2 t = A[r * m + c];
-        A[r * m + c] = A[r * m + pr];
-        A[r * m + pr] = t;
-      }
-    __syncthreads();
-  }
-  typedef typename SemZ<Z>::R R;
-  Z* out = ainv + ((int64_t)w * 2 + sp) * mm * mm;
-  for (int e = tid; e < m * m; e += 256) {
-    const int r = e / m, j = e % m;
-    out[r * mm + j] = Z{(R)A[e].x, (R)A[e].y};
-  }
+}
+
+template <typename Z, int MR>
+static int launch_sem_inverse(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, int sp, int mm,
+                              hipStream_t st) {
+  hipLaunchKernelGGL((k_sem_inverse<Z, MR, 4>), dim3(w->nw), dim3(MR * 4), 0, st, c->s, sp, w->x, (Z*)d->ainv, mm);
+  HIPCHK(hipGetLastError());
+  return QMC_OK;
+}
+
+template <typename Z>
+static int sem_inverse_spin(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, int sp, int mm,
+                            hipStream_t st) {
+  const int m = c->s.np[sp];
+  if (m <= 16) return launch_sem_inverse<Z, 16>(c, w, d, sp, mm, st);
+  if (m <= 32) return launch_sem_inverse<Z, 32>(c, w, d, sp, mm, st);
+  return launch_sem_inverse<Z, 64>(c, w, d, sp, mm, st);
 }
 
 // Moves of electrons j0..j1-1 of spin sp, in order: one workgroup of
@@ -397,19 +416,12 @@ extern "C" int qmc_sem_refresh(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_s
   if (w->nw == 0) return QMC_OK;
   HIPCHK(hipSetDevice(c->device));
   const int mm = sem_mm(c);
-  const size_t lds = sizeof(double2) * (size_t)mm * mm;
-  if (lds > 65536)
-    HIPCHK(hipFuncSetAttribute(precision == QMC_PREC_MIXED ? (const void*)k_sem_inverse<float2>
-                                                           : (const void*)k_sem_inverse<double2>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const dim3 grid(w->nw, c->s.ntypes);
-  if (precision == QMC_PREC_MIXED)
-    hipLaunchKernelGGL(k_sem_inverse<float2>, grid, dim3(256), lds, (hipStream_t)stream, c->s, w->nw, w->x,
-                       (float2*)d->ainv, mm);
-  else
-    hipLaunchKernelGGL(k_sem_inverse<double2>, grid, dim3(256), lds, (hipStream_t)stream, c->s, w->nw, w->x,
-                       (double2*)d->ainv, mm);
-  HIPCHK(hipGetLastError());
+  for (int sp = 0; sp < c->s.ntypes; sp++) {
+    if (c->s.np[sp] == 0) continue;
+    rc = precision == QMC_PREC_MIXED ? sem_inverse_spin<float2>(c, w, d, sp, mm, (hipStream_t)stream)
+                                     : sem_inverse_spin<double2>(c, w, d, sp, mm, (hipStream_t)stream);
+    if (rc) return rc;
+  }
   return QMC_OK;
 }
 
