@@ -1,7 +1,7 @@
 // qmc_sem.hip -- single-electron VMC moves with Sherman-Morrison updates of
 // the inverse Slater matrices, for trial functions without backflow (every
-// backflow table zero, so quasi-coordinates are positions) and with the
-// two-body Jastrow u2.  The reference has no such move: its move displaces
+// backflow table zero, so quasi-coordinates are positions), with the two-
+// and three-body Jastrow factors.  The reference has no such move: its move displaces
 // every electron and recomputes everything (reptation.f:4617-4680); this is
 // the single-electron scheme of the plain Slater-Jastrow function, checked
 // against the full evaluation (the log|psi| change of every accepted move)
@@ -11,8 +11,11 @@
 // the spin) of A(k, j) = exp(i g_k . r_j) to a' and nothing else, so the
 // determinant ratio is R = (A^-1 a')_j' and
 // A'^-1 = A^-1 - (A^-1 a' - e_j') (row j' of A^-1) / R.  The Jastrow ratio is
-// exp(-dU), dU = sum_p u(|r' - r_p|) - u(|r - r_p|).  The move is accepted
-// with probability |R|^2 exp(-2 dU) (a symmetric Gaussian proposal).
+// exp(-dU), dU = sum_p u(|r' - r_p|) - u(|r - r_p|) plus, with u3, the change
+// of the three-body term sum_pairs (xi r)^2 - 1/2 sum_p |G_p|^2 of the full
+// evaluation, G_p = sum_b xi(r_pb) (x_b - x_p), which needs only the O(N)
+// changed pair terms and the stored G.  The move is accepted with
+// probability |R|^2 exp(-2 dU) (a symmetric Gaussian proposal).
 //
 // The inverses live in HBM between calls ([nw][2][mm][mm] complex, row =
 // particle, column = orbital), in FP64 or, with QMC_PREC_MIXED, in FP32 (the
@@ -231,21 +234,23 @@ struct SemWaves {
   static constexpr int value = (sizeof(Z) == 16 && MR == 64) ? 3 : 1;
 };
 
-template <typename Z, int MR, int CG>
+template <typename Z, int MR, int CG, bool U3>
 __global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves(DevSys s, int j0, int j1, int sp, double sigma,
                                                       double* __restrict__ x, int64_t* __restrict__ rng,
                                                       Z* __restrict__ ainv, int mm, double* __restrict__ dlogpsi,
                                                       double* __restrict__ nacc) {
   typedef typename SemZ<Z>::R R;
   constexpr int T = MR * CG, EPT = MR / CG, NWV = T / 64 > 0 ? T / 64 : 1;
+  constexpr int PMAX = (128 + T - 1) / T;  // partners per thread (N <= 128)
   __shared__ Z ap[MR];    // new orbital column a'
   __shared__ Z rowj[MR];  // row j' of the inverse before the update
   __shared__ Z rz_sh;     // R
-  __shared__ double red[NWV];
+  __shared__ double red[3 * NWV];
   __shared__ double2 step[64];  // move i's displacement
   __shared__ double uacc[64];   // move i's acceptance uniform
   extern __shared__ __attribute__((aligned(16))) unsigned char sem_lds[];
   double* xs = reinterpret_cast<double*>(sem_lds);  // positions [N][2]
+  double* Gs = xs + 2 * s.nptot;                     // U3: G_p [N][2]
   const int w = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int N = s.nptot, nmv = j1 - j0;
   const int m = pick2(s.np, sp), p0 = pick2(s.ipf, sp);
@@ -272,6 +277,27 @@ __global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves(
   }
   double du_acc = 0.0, na = 0.0, pm = 1.0;
   int pe = 0;
+  if (U3) {
+    // G_p = sum_b xi(r_pb) (x_b - x_p), minimum image: the per-particle
+    // vector of the three-body factor
+    __syncthreads();
+    for (int p = tid; p < N; p += T) {
+      double gx = 0.0, gy = 0.0;
+      const int tp = ptype(s, p);
+      for (int q = 0; q < N; q++) {
+        if (q == p) continue;
+        const TabRef tu3 = tabref<TK_U3>(s, tpair(tp, ptype(s, q)));
+        if (tu3.row0 < 0) continue;
+        const PairGeo g = pair_geo(s, xs[2 * q], xs[2 * q + 1], xs[2 * p], xs[2 * p + 1]);
+        double f, d1, d2;
+        spline3(s, tu3, g.ind, g.rem, f, d1, d2);
+        gx += f * g.ux;
+        gy += f * g.uy;
+      }
+      Gs[2 * p] = gx;
+      Gs[2 * p + 1] = gy;
+    }
+  }
   __syncthreads();
   for (int j = j0; j < j1; j++) {
     const int jj = j - p0;
@@ -313,25 +339,64 @@ __global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves(
       for (int i = 0; i < EPT; i++) rowj[cg + CG * i] = b[i];
       if (cg == 0) rz_sh = Z{tr, ti};
     }
-    // the Jastrow change over the partners of j
-    double du = 0.0;
-    for (int p = tid; p < N; p += T) {
-      if (p == j) continue;
-      const TabRef tu2 = tabref<TK_U2>(s, tpair(sp, ptype(s, p)));
-      if (tu2.row0 < 0) continue;
+    // the Jastrow change over the partners of j: the u2 terms and, with U3,
+    // the three-body factor's pair terms (xi r)^2 and its -1/2 sum_p |G_p|^2
+    // (G_p of each partner changes by dG_p = xi' u' - xi u, u = x_j - x_p
+    // folded; G_j becomes -sum_p xi' u')
+    double du = 0.0, gjx = 0.0, gjy = 0.0;
+    double dgx[PMAX], dgy[PMAX];
+    const double2 Gj = U3 ? ld2(Gs, j) : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < PMAX; k++) {
+      const int p = tid + k * T;
+      dgx[k] = dgy[k] = 0.0;
+      if (p >= N || p == j) continue;
+      const int tp = tpair(sp, ptype(s, p));
+      const TabRef tu2 = tabref<TK_U2>(s, tp);
       const double xp = xs[2 * p], yp = xs[2 * p + 1];
       const PairGeo pn = pair_geo(s, xn, yn, xp, yp), po = pair_geo(s, xo, yo, xp, yp);
       double fn, fo, d1, d2;
-      spline3(s, tu2, pn.ind, pn.rem, fn, d1, d2);
-      spline3(s, tu2, po.ind, po.rem, fo, d1, d2);
-      du += fn - fo;
+      if (tu2.row0 >= 0) {
+        spline3(s, tu2, pn.ind, pn.rem, fn, d1, d2);
+        spline3(s, tu2, po.ind, po.rem, fo, d1, d2);
+        du += fn - fo;
+      }
+      if (U3) {
+        const TabRef tu3 = tabref<TK_U3>(s, tp);
+        if (tu3.row0 >= 0) {
+          spline3(s, tu3, pn.ind, pn.rem, fn, d1, d2);
+          spline3(s, tu3, po.ind, po.rem, fo, d1, d2);
+          const double an = fn * pn.r, ao = fo * po.r;
+          dgx[k] = fn * pn.ux - fo * po.ux;
+          dgy[k] = fn * pn.uy - fo * po.uy;
+          const double2 G = ld2(Gs, p);
+          du += (an * an - ao * ao) - (G.x * dgx[k] + G.y * dgy[k]) - 0.5 * (dgx[k] * dgx[k] + dgy[k] * dgy[k]);
+          gjx -= fn * pn.ux;
+          gjy -= fn * pn.uy;
+        }
+      }
     }
     du = wave_sum(du);
-    if (lane == 0) red[wv] = du;
+    if (U3) {
+      gjx = wave_sum(gjx);
+      gjy = wave_sum(gjy);
+    }
+    if (lane == 0) {
+      red[wv] = du;
+      red[NWV + wv] = gjx;
+      red[2 * NWV + wv] = gjy;
+    }
     __syncthreads();
     du = 0.0;
+    gjx = 0.0;
+    gjy = 0.0;
 #pragma unroll
-    for (int k = 0; k < NWV; k++) du += red[k];
+    for (int k = 0; k < NWV; k++) {
+      du += red[k];
+      gjx += red[NWV + k];
+      gjy += red[2 * NWV + k];
+    }
+    if (U3) du -= 0.5 * ((gjx * gjx + gjy * gjy) - (Gj.x * Gj.x + Gj.y * Gj.y));
     const Z rz = rz_sh;
     const double mod2 = (double)rz.x * (double)rz.x + (double)rz.y * (double)rz.y;
     if (uacc[j - j0] < mod2 * sem_exp(-2.0 * du)) {
@@ -350,6 +415,20 @@ __global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves(
       if (tid == 0) {
         xs[2 * j] = xn;
         xs[2 * j + 1] = yn;
+      }
+      if (U3) {
+#pragma unroll
+        for (int k = 0; k < PMAX; k++) {
+          const int p = tid + k * T;
+          if (p < N && p != j) {
+            Gs[2 * p] += dgx[k];
+            Gs[2 * p + 1] += dgy[k];
+          }
+        }
+        if (tid == 0) {
+          Gs[2 * j] = gjx;
+          Gs[2 * j + 1] = gjy;
+        }
       }
       int e;
       pm = frexp(pm * mod2, &e);
@@ -377,10 +456,14 @@ template <typename Z, int MR>
 static int launch_sem_moves(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, int j0, int j1, int sp,
                             double sigma, int mm, hipStream_t st) {
   constexpr int CG = 4;
-  const size_t lds = 2 * (size_t)c->s.nptot * sizeof(double);
-  if (lds > 65536) return arg_fail("too many particles for the single-electron move kernel", QMC_ERR_LIMIT);
-  hipLaunchKernelGGL((k_sem_moves<Z, MR, CG>), dim3(w->nw), dim3(MR * CG), lds, st, c->s, j0, j1, sp, sigma,
-                     w->x, w->rng, (Z*)d->ainv, mm, d->dlogpsi, d->nacc);
+  const bool u3 = c->s.has_u3 != 0;
+  const size_t lds = (u3 ? 4 : 2) * (size_t)c->s.nptot * sizeof(double);
+  if (u3)
+    hipLaunchKernelGGL((k_sem_moves<Z, MR, CG, true>), dim3(w->nw), dim3(MR * CG), lds, st, c->s, j0, j1, sp,
+                       sigma, w->x, w->rng, (Z*)d->ainv, mm, d->dlogpsi, d->nacc);
+  else
+    hipLaunchKernelGGL((k_sem_moves<Z, MR, CG, false>), dim3(w->nw), dim3(MR * CG), lds, st, c->s, j0, j1, sp,
+                       sigma, w->x, w->rng, (Z*)d->ainv, mm, d->dlogpsi, d->nacc);
   HIPCHK(hipGetLastError());
   return QMC_OK;
 }
@@ -400,7 +483,6 @@ static int sem_check(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, i
     return arg_fail("null buffer for single-electron moves");
   if (precision != QMC_PREC_FP64 && precision != QMC_PREC_MIXED) return arg_fail("unknown precision mode");
   if (!c->b_zero) return arg_fail("single-electron moves need a trial function without backflow");
-  if (c->s.has_u3) return arg_fail("single-electron moves support the two-body Jastrow only");
   if (!c->s.has_det) return arg_fail("single-electron moves need the Slater determinants");
   if (std::max(c->s.np[0], c->s.ntypes > 1 ? c->s.np[1] : 0) > QMC_MAX_ORB)
     return arg_fail("single-electron moves support at most 64 particles per spin", QMC_ERR_LIMIT);

@@ -384,9 +384,10 @@ int qmc_dmc_branch(qmc_ctx* ctx, int nw, double* d_wate, int64_t* d_rng2, int32_
  * Single-electron VMC moves with Sherman-Morrison updates of the inverse
  * Slater matrices.  No counterpart in the reference, whose move displaces
  * every electron at once (move, reptation.f:4617-4680); for trial functions
- * without backflow and three-body Jastrow (every backflow table zero, no
- * u3; other decks return QMC_ERR_ARG), where moving one electron changes
- * one column of its spin's matrix A(k, j) = exp(i g_k . r_j).
+ * without backflow (every backflow table zero; other decks return
+ * QMC_ERR_ARG), where moving one electron changes one column of its spin's
+ * matrix A(k, j) = exp(i g_k . r_j).  The two- and three-body Jastrow
+ * factors are both supported.
  *
  * qmc_sem_state: device buffers owned by the caller, for w->nw walkers.
  * ainv holds per walker and spin the inverse, row = particle, column =
@@ -410,7 +411,8 @@ int qmc_sem_refresh(qmc_ctx* ctx, const qmc_walkers* w, const qmc_sem_state* d, 
  * walker: a Gaussian step of width sigma per dimension (Box-Muller from the
  * walker's rannyu stream w->rng: two draws for the step, a third for the
  * acceptance), accepted with probability |R|^2 exp(-2 dU), where R is the
- * determinant ratio (A^-1 a')_j and dU the change of the u2 sum.  Positions
+ * determinant ratio (A^-1 a')_j and dU the change of the Jastrow exponent
+ * (the u2 sum and the three-body term of the full evaluation).  Positions
  * and inverses are updated in place; w->props, w->g, w->h are not.
  */
 int qmc_sem_moves(qmc_ctx* ctx, const qmc_walkers* w, const qmc_sem_state* d, int j0, int j1,

@@ -121,9 +121,61 @@ def test_split_ranges_replay_the_same_walk():
             assert np.array_equal(u, v), k
 
 
+def _u3_deck(tmp_path, nup=13, ndown=13, rs=5.0, shells=12):
+    dg = pkg("deckgen")
+    dg.make_deck(str(tmp_path), "u3z", nup, ndown, rs, 5, nshell_k=shells, u3=True, backflow="zero",
+                 jastrow="pade")
+    return pkg("deck").read_deck(str(tmp_path))
+
+
+def test_sweeps_with_three_body_jastrow_exact(tmp_path):
+    """u2 + u3, zero backflow (generated N=26 deck): the summed
+    log|R| - dU of the accepted moves, dU with the three-body term carried
+    through the stored per-particle vectors G, equals -(ltf_after -
+    ltf_before) of full evaluations, and the inverse stays exact."""
+    print("\n[gpu] sem sweeps with u3 (generated N=26 deck)", flush=True)
+    import torch
+    d = _u3_deck(tmp_path)
+    d, ctx, wk, sm = _setup(None, 64, deck=d)
+    assert ctx.info["has_det"]
+    jltf = ctx.info["jltf"] - 1
+    ltf0 = wk.props[:, jltf].clone()
+    sm.refresh()
+    for _ in range(5):
+        sm.sweep(0.5)
+    worst = _check_inverse_exact(d, ctx, wk, sm, 1e-9)
+    wk.start()
+    torch.cuda.synchronize()
+    err = float(np.max(np.abs(sm.dlogpsi.cpu().numpy() + (wk.props[:, jltf] - ltf0).cpu().numpy())))
+    acc = float(sm.nacc.sum().item()) / (64 * 5 * d.nptot)
+    print(f"   inverse rel err {worst:.2e}  |dlogpsi - full| {err:.2e}  acceptance {acc:.3f}", flush=True)
+    assert 0.2 < acc < 0.9
+    assert err < 1e-9
+
+
+def test_energy_with_three_body_agrees_with_all_electron_walk(tmp_path):
+    """Statistical agreement with u3 (generated N=26 deck, zero backflow):
+    the single-electron walk against the all-electron walk with the drift
+    cutoff."""
+    print("\n[gpu] sem energy with u3 vs all-electron walk", flush=True)
+    d = _u3_deck(tmp_path)
+    d.adrift = 0.5
+    hip = pkg("hip")
+    ctx = hip.Context(d, twist=(1, 1))
+    rs2 = 1.0 / d.hbs2m[0]
+    ea, nmove = _energy_all_electron(ctx, d, 512, 0.01 * rs2, 400, 2000)
+    es, acc = _energy_single(ctx, d, 512, 0.5, 40, 200)
+    ma, sa = _blocked(ea)
+    ms, ss = _blocked(es)
+    z = abs(ma - ms) / np.hypot(sa, ss)
+    print(f"   all-electron {ma:.6f} +- {sa:.6f} ({nmove} of 512 walkers move)   single-electron "
+          f"{ms:.6f} +- {ss:.6f} (acc {acc:.3f})  z {z:.2f}", flush=True)
+    assert nmove == 512
+    assert z < 4.5
+
+
 def test_moves_range_and_deck_checks():
-    """Electron ranges out of bounds and decks with backflow or a
-    three-body Jastrow are refused."""
+    """Electron ranges out of bounds and decks with backflow are refused."""
     print("\n[gpu] sem argument checks", flush=True)
     hip = pkg("hip")
     d, ctx, wk, sm = _setup("c26", 4)
