@@ -356,6 +356,52 @@ class SingleMoves:
             if bounds[t + 1] > bounds[t]:
                 self.moves(bounds[t], bounds[t + 1], sigma, stream)
 
+    def advance(self, nmoves, sigma, stream=None):
+        """nmoves moves per walker, continuing through the electrons in order
+        from where the previous advance stopped (wrapping at N)."""
+        n = int(self.wk.ctx.deck.nptot)
+        cur = getattr(self, "_cursor", 0)
+        left = int(nmoves)
+        while left > 0:
+            b = min(n, cur + left)
+            self.moves(cur, b, sigma, stream)
+            left -= b - cur
+            cur = b % n
+        self._cursor = cur
+
+    def drift(self):
+        """Largest relative distance, over walkers and spins, between the
+        carried inverses and fresh FP64 inverses of the current positions
+        (the carried ones are then replaced by the fresh ones: this is a
+        refresh that also measures)."""
+        import torch
+        old = self.ainv.to(torch.float64, copy=True)
+        self.refresh()
+        new = self.ainv.to(torch.float64)
+        num = (old - new).abs().amax(dim=(2, 3, 4))
+        den = new.abs().amax(dim=(2, 3, 4)).clamp_min(1e-300)
+        live = new.abs().amax(dim=(2, 3, 4)) > 0
+        return float((num / den)[live].max().item())
+
+    def tune_refresh(self, sigma, candidates=(16, 64, 256), tol=1e-4, cycles=3, stream=None):
+        """Refresh interval K (moves per walker between FP64 refreshes): for
+        each candidate, `cycles` rounds of a refresh, K moves and a measured
+        refresh (drift); returns the largest K whose worst drift stays
+        within tol (the smallest candidate if none does) and the drifts.
+        The default tol, 1e-4 relative on the largest entry of any walker's
+        inverse, bounds the ratio R = (A^-1 a')_j to about that relative
+        error; FP32 arithmetic alone leaves some 1e-6 there."""
+        drifts = {}
+        for k in sorted(candidates):
+            self.refresh(stream)
+            worst = 0.0
+            for _ in range(cycles):
+                self.advance(k, sigma, stream)
+                worst = max(worst, self.drift())
+            drifts[int(k)] = worst
+        ok = [k for k in sorted(drifts) if drifts[k] <= tol]
+        return (ok[-1] if ok else min(drifts)), drifts
+
 
 # fixed fields of one reptation trace row before the per-bead probabilities
 RMC_TRACE_FIELDS = ("idir", "ndelta", "wnew", "age", "e", "ltf", "kin", "spare")

@@ -437,7 +437,10 @@ def bench_sem(args, hip, deck, dg):
     """Single-electron sweeps/s (qmc_sem_moves over every electron, one spin
     per launch) with FP64 and FP32 inverses, beside the inverse refresh, the
     full evaluation that gives a sweep's local energy, and one all-electron
-    VMC step on the same deck and walkers."""
+    VMC step on the same deck and walkers; then N=58 with u2 + u3 at rs 1, 5
+    and 10 (8192 walkers), where the FP32 inverses' refresh interval K is
+    tuned against the measured drift and the log|psi| carried over K moves
+    is checked against full evaluations."""
     import torch
     abi = importlib.import_module("2degas_amd.abi")
     res = {}
@@ -481,6 +484,51 @@ def bench_sem(args, hip, deck, dg):
         delta = 0.05 * rs * rs / d.nptot
         wk.start(delta=delta)
         r["all_electron_step_ms"] = timed(lambda: wk.sweep(1, delta), 10)
+        res[tag] = r
+        wk = None
+        ctx.close()
+    # the batched plain-vs-mixed shape: N=58 (29+29), u2 + u3, zero backflow,
+    # 8192 walkers, step 0.5 rs, rs 1, 5, 10; FP32 inverses refreshed in FP64
+    # every K moves, K tuned against the measured drift
+    for rs in (1.0, 5.0, 10.0):
+        tag = f"n58_rs{rs:g}"
+        tmp = tempfile.mkdtemp(prefix="degas_sem_")
+        deckdir = os.path.join(tmp, "deck")
+        dg.make_deck(deckdir, "n58", 29, 29, rs, 60, u3=True, backflow="zero", jastrow="pade")
+        d = deck.read_deck(deckdir)
+        nw = 8192
+        x = dg.lattice_configs(nw, d.nptot, d.el[0], 58)
+        sts = [abi.limbs_to_state([3535, 3467, 3262, (2 * (1549 + k) + 1) % 4096]) for k in range(nw)]
+        ctx = hip.Context(d, twist=(1, 1), device=0)
+        wk = hip.Walkers(ctx, x, sts)
+        wk.start()
+        jltf = ctx.info["jltf"] - 1
+        r = {"N": d.nptot, "walkers": nw, "rs": rs, "sigma": 0.5}
+        for prec, name in ((abi.QMC_PREC_FP64, "fp64"), (abi.QMC_PREC_MIXED, "mixed")):
+            sm = hip.SingleMoves(wk, precision=prec)
+            sm.refresh()
+            for _ in range(max(args.warmup, 5)):
+                sm.sweep(0.5)
+            sm.nacc.zero_()
+            ms = timed(lambda: sm.sweep(0.5), max(args.steps, 5))
+            acc = float(sm.nacc.sum().item()) / (nw * d.nptot * (max(args.steps, 5) + 1))
+            rms = timed(sm.refresh, 3)
+            kbest, drifts = sm.tune_refresh(0.5)
+            # log|psi| carried over K moves against full evaluations
+            sm.refresh()
+            wk.start()
+            ltf0 = wk.props[:, jltf].clone()
+            sm.dlogpsi.zero_()
+            sm.advance(kbest, 0.5)
+            wk.start()
+            torch.cuda.synchronize()
+            lerr = float((sm.dlogpsi + (wk.props[:, jltf] - ltf0)).abs().max().item())
+            r[name] = {"sweep_ms": ms, "refresh_ms": rms, "acceptance": acc, "K": kbest,
+                       "drift_by_K": drifts, "logpsi_err_after_K": lerr,
+                       "sweep_ms_with_refresh": ms + rms * d.nptot / kbest,
+                       "electron_moves_per_s": 1e3 / ms * nw * d.nptot}
+            sm = None
+        r["eval_ms"] = timed(wk.start, 5)
         res[tag] = r
         wk = None
         ctx.close()

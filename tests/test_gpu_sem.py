@@ -174,6 +174,24 @@ def test_energy_with_three_body_agrees_with_all_electron_walk(tmp_path):
     assert z < 4.5
 
 
+def test_refresh_tuner_measures_drift():
+    """drift() compares the carried inverses with fresh FP64 ones: rounding
+    only in FP64, FP32-sized in mixed precision, and growing with the moves
+    between refreshes; tune_refresh returns one of its candidates."""
+    print("\n[gpu] sem refresh tuner c26", flush=True)
+    d, ctx, wk, sm = _setup("c26", 64)
+    sm.refresh()
+    sm.advance(40, 0.5)
+    d64 = sm.drift()
+    d, ctx, wk, sm = _setup("c26", 64, precision=1)
+    k, drifts = sm.tune_refresh(0.5, candidates=(8, 64, 256), cycles=2)
+    print(f"   FP64 drift after 40 moves {d64:.1e}; FP32 drifts {drifts} -> K {k}", flush=True)
+    assert d64 < 1e-12
+    assert k in (8, 64, 256)
+    assert 1e-9 < drifts[8] < 1e-3
+    assert drifts[256] < 1e-2
+
+
 def test_moves_range_and_deck_checks():
     """Electron ranges out of bounds and decks with backflow are refused."""
     print("\n[gpu] sem argument checks", flush=True)
