@@ -479,13 +479,99 @@ static int sem_moves_spin(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state*
 
 static int sem_check(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, int precision) {
   if (!c || !w || !d || w->nw < 0) return arg_fail("null argument");
-  if (w->nw > 0 && (!w->x || !w->rng || !d->ainv || !d->dlogpsi || !d->nacc))
+  if (w->nw > 0 && (!w->x || !w->rng || !d->dlogpsi || !d->nacc))
     return arg_fail("null buffer for single-electron moves");
   if (precision != QMC_PREC_FP64 && precision != QMC_PREC_MIXED) return arg_fail("unknown precision mode");
-  if (!c->b_zero) return arg_fail("single-electron moves need a trial function without backflow");
   if (!c->s.has_det) return arg_fail("single-electron moves need the Slater determinants");
-  if (std::max(c->s.np[0], c->s.ntypes > 1 ? c->s.np[1] : 0) > QMC_MAX_ORB)
-    return arg_fail("single-electron moves support at most 64 particles per spin", QMC_ERR_LIMIT);
+  if (c->b_zero) {
+    if (w->nw > 0 && !d->ainv) return arg_fail("null inverse buffer for single-electron moves");
+    if (std::max(c->s.np[0], c->s.ntypes > 1 ? c->s.np[1] : 0) > QMC_MAX_ORB)
+      return arg_fail("single-electron moves support at most 64 particles per spin", QMC_ERR_LIMIT);
+  } else if (w->nw > 0 && (!w->g || !w->h || !w->props || !w->grad2_ph)) {
+    return arg_fail("null walker buffer: moves with backflow keep the full evaluation");
+  }
+  return QMC_OK;
+}
+
+// With backflow every quasi-coordinate depends on every electron, so one
+// electron's move changes every column of both orbital matrices and no
+// low-rank update applies: each move is a full evaluation of the proposed
+// configuration (the one the reference's compute_properties performs),
+// accepted with probability |psi'/psi|^2 = exp(-2 (ltf' - ltf)).  One
+// wavefront per walker copies the walker with electron j displaced (the
+// same three draws per move as the update path); after the evaluation one
+// wavefront per walker takes the decision and copies the proposal's
+// properties, drift and Laplacian sums into the walker when accepted.
+__global__ __launch_bounds__(256) void k_semf_propose(DevSys s, int nw, int j, double sigma,
+                                                      const double* __restrict__ x, double* __restrict__ xn,
+                                                      int64_t* __restrict__ rng, double* __restrict__ ua) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = blockIdx.x * 4 + wv;
+  if (w >= nw) return;
+  const int N = s.nptot;
+  const double* xw = x + (int64_t)w * 2 * N;
+  double* xo = xn + (int64_t)w * 2 * N;
+  for (int e = lane; e < 2 * N; e += 64)
+    if (e != 2 * j && e != 2 * j + 1) xo[e] = xw[e];
+  if (lane == 0) {
+    uint64_t st = (uint64_t)rng[w];
+    const double u1 = rannyu(st), u2 = rannyu(st), a = rannyu(st);
+    double sv, cv;
+    qsincos(6.283185307179586 * u2, &sv, &cv);
+    const double rad = sigma * sqrt(-2.0 * log(u1));
+    xo[2 * j] = xw[2 * j] + rad * cv;
+    xo[2 * j + 1] = xw[2 * j + 1] + rad * sv;
+    rng[w] = (int64_t)st;
+    ua[w] = a;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_semf_accept(DevSys s, int nw, int j, int jltf, const double* __restrict__ xn,
+                                                     const double* __restrict__ pn, const double* __restrict__ gn,
+                                                     const double* __restrict__ hn, const int32_t* __restrict__ stn,
+                                                     const double* __restrict__ ua, double* __restrict__ x,
+                                                     double* __restrict__ props, double* __restrict__ g,
+                                                     double* __restrict__ h, double* __restrict__ dlogpsi,
+                                                     double* __restrict__ nacc) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = blockIdx.x * 4 + wv;
+  if (w >= nw) return;
+  const int N = s.nptot, NP = s.n_props;
+  const double lo = props[(int64_t)w * NP + jltf], ln = pn[(int64_t)w * NP + jltf];
+  // a singular proposal (status bit 2) has psi' = 0; the underflow guard
+  // (bit 1) leaves ltf valid, and the decision reads ltf as metropolis_test
+  // does
+  if ((stn[w] & 2) != 0 || !(ua[w] < exp(-2.0 * (ln - lo)))) return;
+  for (int e = lane; e < NP; e += 64) props[(int64_t)w * NP + e] = pn[(int64_t)w * NP + e];
+  for (int e = lane; e < 2 * N; e += 64) g[(int64_t)w * 2 * N + e] = gn[(int64_t)w * 2 * N + e];
+  if (lane < 2) {
+    h[2 * w + lane] = hn[2 * w + lane];
+    x[(int64_t)w * 2 * N + 2 * j + lane] = xn[(int64_t)w * 2 * N + 2 * j + lane];
+  }
+  if (lane == 0) {
+    dlogpsi[w] += lo - ln;
+    nacc[w] += 1.0;
+  }
+}
+
+static int sem_moves_full(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, int j0, int j1, double sigma,
+                          int precision, hipStream_t st) {
+  const int nw = w->nw;
+  int rc = ensure_vmc_scratch(c, nw);
+  if (rc) return rc;
+  rc = ensure_ws(c, nw);
+  if (rc) return rc;
+  const dim3 grid((nw + 3) / 4);
+  for (int j = j0; j < j1; j++) {
+    hipLaunchKernelGGL(k_semf_propose, grid, dim3(256), 0, st, c->s, nw, j, sigma, w->x, c->d_xn, w->rng,
+                       c->d_wold);
+    // the fixed-phase scratch is shared by proposals, as in the VMC step
+    rc = launch_eval(c, nw, c->d_xn, c->d_pn, c->d_gn, c->d_hn, w->grad2_ph, c->d_stn, st, c->delta, precision);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_semf_accept, grid, dim3(256), 0, st, c->s, nw, j, c->info.jltf - 1, c->d_xn, c->d_pn,
+                       c->d_gn, c->d_hn, c->d_stn, c->d_wold, w->x, w->props, w->g, w->h, d->dlogpsi, d->nacc);
+  }
+  HIPCHK(hipGetLastError());
   return QMC_OK;
 }
 
@@ -495,7 +581,7 @@ extern "C" int qmc_sem_refresh(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_s
                                void* stream) {
   int rc = sem_check(c, w, d, precision);
   if (rc) return rc;
-  if (w->nw == 0) return QMC_OK;
+  if (w->nw == 0 || !c->b_zero) return QMC_OK;  // backflow: no inverse is carried
   HIPCHK(hipSetDevice(c->device));
   const int mm = sem_mm(c);
   for (int sp = 0; sp < c->s.ntypes; sp++) {
@@ -515,6 +601,7 @@ extern "C" int qmc_sem_moves(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_sta
   if (!(sigma > 0.0)) return arg_fail("sigma must be positive");
   if (w->nw == 0 || j0 == j1) return QMC_OK;
   HIPCHK(hipSetDevice(c->device));
+  if (!c->b_zero) return sem_moves_full(c, w, d, j0, j1, sigma, precision, (hipStream_t)stream);
   const int mm = sem_mm(c);
   // one launch per spin the range touches, in order
   for (int sp = 0; sp < c->s.ntypes; sp++) {

@@ -383,11 +383,15 @@ int qmc_dmc_branch(qmc_ctx* ctx, int nw, double* d_wate, int64_t* d_rng2, int32_
 /*
  * Single-electron VMC moves with Sherman-Morrison updates of the inverse
  * Slater matrices.  No counterpart in the reference, whose move displaces
- * every electron at once (move, reptation.f:4617-4680); for trial functions
- * without backflow (every backflow table zero; other decks return
- * QMC_ERR_ARG), where moving one electron changes one column of its spin's
- * matrix A(k, j) = exp(i g_k . r_j).  The two- and three-body Jastrow
- * factors are both supported.
+ * every electron at once (move, reptation.f:4617-4680).  For trial
+ * functions without backflow (every backflow table zero) moving one
+ * electron changes one column of its spin's matrix A(k, j) = exp(i g_k . r_j)
+ * and the inverse is updated (the two- and three-body Jastrow factors are
+ * both supported).  With backflow every quasi-coordinate moves with the
+ * electron, no low-rank update applies, and each move is a full evaluation
+ * of the proposed configuration: there ainv is not used (may be NULL),
+ * qmc_sem_refresh does nothing, and the walkers' props, g, h and grad2_ph
+ * must hold the current evaluation (qmc_vmc_start) and are kept current.
  *
  * qmc_sem_state: device buffers owned by the caller, for w->nw walkers.
  * ainv holds per walker and spin the inverse, row = particle, column =
@@ -412,8 +416,10 @@ int qmc_sem_refresh(qmc_ctx* ctx, const qmc_walkers* w, const qmc_sem_state* d, 
  * walker's rannyu stream w->rng: two draws for the step, a third for the
  * acceptance), accepted with probability |R|^2 exp(-2 dU), where R is the
  * determinant ratio (A^-1 a')_j and dU the change of the Jastrow exponent
- * (the u2 sum and the three-body term of the full evaluation).  Positions
- * and inverses are updated in place; w->props, w->g, w->h are not.
+ * (the u2 sum and the three-body term of the full evaluation); with
+ * backflow, with probability exp(-2 (ltf' - ltf)) from the evaluation of
+ * the proposal.  Positions and inverses are updated in place; w->props,
+ * w->g, w->h are not (without backflow) or are (with backflow).
  */
 int qmc_sem_moves(qmc_ctx* ctx, const qmc_walkers* w, const qmc_sem_state* d, int j0, int j1,
                   double sigma, int precision, void* stream);

@@ -192,8 +192,9 @@ def test_refresh_tuner_measures_drift():
     assert drifts[256] < 1e-2
 
 
-def test_moves_range_and_deck_checks():
-    """Electron ranges out of bounds and decks with backflow are refused."""
+def test_moves_range_checks():
+    """Electron ranges out of bounds and a step that is not positive are
+    refused."""
     print("\n[gpu] sem argument checks", flush=True)
     hip = pkg("hip")
     d, ctx, wk, sm = _setup("c26", 4)
@@ -203,9 +204,62 @@ def test_moves_range_and_deck_checks():
         sm.moves(3, 2, 0.5)
     with pytest.raises(hip.QmcError):
         sm.moves(0, 4, 0.0)
-    _, _, wk2, sm2 = _setup("n26", 4)
-    with pytest.raises(hip.QmcError):
-        sm2.refresh()
+
+
+@pytest.mark.parametrize("name", ["n26g", "g58"])
+def test_backflow_moves_keep_the_full_evaluation(name):
+    """With backflow (u2 + u3 + backflow decks) each move is a full
+    evaluation of the proposal: after three sweeps the walkers' properties,
+    drift and Laplacian sums equal a fresh evaluation of their positions,
+    and the summed log|psi| change equals -(ltf_after - ltf_before)."""
+    print(f"\n[gpu] sem backflow moves {name}", flush=True)
+    import torch
+    d, ctx, wk, sm = _setup(name, 32)
+    jltf = ctx.info["jltf"] - 1
+    ltf0 = wk.props[:, jltf].clone()
+    sm.refresh()
+    for _ in range(3):
+        sm.sweep(0.5)
+    torch.cuda.synchronize()
+    kept = wk.props.cpu().numpy().copy(), wk.g.cpu().numpy().copy(), wk.h.cpu().numpy().copy()
+    dl = sm.dlogpsi.cpu().numpy()
+    acc = float(sm.nacc.sum().item()) / (32 * 3 * d.nptot)
+    wk.start()
+    torch.cuda.synchronize()
+    fresh = wk.props.cpu().numpy(), wk.g.cpu().numpy(), wk.h.cpu().numpy()
+    # every slot but the acceptance slot, which the start sets to 1
+    keep = np.arange(kept[0].shape[1]) != ctx.info["jacc"] - 1
+    kept = (kept[0][:, keep], kept[1], kept[2])
+    fresh = (fresh[0][:, keep], fresh[1], fresh[2])
+    errs = [float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)) for a, b in zip(kept, fresh)]
+    lerr = float(np.max(np.abs(dl + (wk.props[:, jltf].cpu().numpy() - ltf0.cpu().numpy()))))
+    print(f"   acceptance {acc:.3f}; kept vs fresh props/g/h {errs}; |dlogpsi - full| {lerr:.1e}", flush=True)
+    assert 0.1 < acc < 0.95
+    assert max(errs) < 1e-12
+    assert lerr < 1e-9
+
+
+def test_energy_with_backflow_agrees_with_all_electron_walk():
+    """Statistical agreement on a backflow deck (n26g: u2 + u3 + the n26
+    backflow, Gamma point, rs=1): the single-electron walk, every move
+    evaluated in full, against the all-electron walk.  At rs=1 the
+    all-electron walk needs the drift cutoff at adrift 1 and a short time
+    step (0.005): with adrift 0.5 or delta 0.01, walkers that reach a node
+    drift off for thousands of steps (block means jumping by 0.05)."""
+    print("\n[gpu] sem energy with backflow vs all-electron walk n26g", flush=True)
+    d = load_deck("n26g")
+    d.adrift = 1.0
+    hip = pkg("hip")
+    ctx = hip.Context(d, twist=(1, 1))
+    ea, nmove = _energy_all_electron(ctx, d, 512, 0.005, 1000, 4000)
+    es, acc = _energy_single(ctx, d, 512, 0.6, 40, 200)
+    ma, sa = _blocked(ea)
+    ms, ss = _blocked(es)
+    z = abs(ma - ms) / np.hypot(sa, ss)
+    print(f"   all-electron {ma:.6f} +- {sa:.6f} ({nmove} of 512 walkers move)   single-electron "
+          f"{ms:.6f} +- {ss:.6f} (acc {acc:.3f})  z {z:.2f}", flush=True)
+    assert nmove >= 500
+    assert z < 4.5
 
 
 def test_mixed_inverse_drift_and_logpsi():
@@ -261,7 +315,7 @@ def _energy_all_electron(ctx, d, nw, delta, neq, nmeas):
     is no longer accepted, and stays for thousands of steps (at N=26 and
     delta 0.25, some 8% of walkers within 2000 steps); the deck is therefore
     run with the reference's drift cutoff (adrift 0.5), which bounds the
-    drift, and the test asks that every walker moves."""
+    drift, and the tests ask that (nearly) every walker moves."""
     import torch
     hip = pkg("hip")
     x = _hard_core_configs(nw, d.nptot, d.el[0], 11)
