@@ -532,6 +532,27 @@ def bench_sem(args, hip, deck, dg):
         res[tag] = r
         wk = None
         ctx.close()
+        # the same shape with the Gaussian backflow of the batched workload
+        # (0.2 exp(-((r - rs)/rs)^2)): every move is a full evaluation
+        tag = f"n58_rs{rs:g}_backflow"
+        deckdir = os.path.join(tmp, "deck_bf")
+        dg.make_deck(deckdir, "n58b", 29, 29, rs, 60, u3=True, backflow="gauss", jastrow="pade")
+        d = deck.read_deck(deckdir)
+        ctx = hip.Context(d, twist=(1, 1), device=0)
+        wk = hip.Walkers(ctx, x, sts)
+        wk.start()
+        r = {"N": d.nptot, "walkers": nw, "rs": rs, "sigma": 0.5}
+        for prec, name in ((abi.QMC_PREC_FP64, "fp64"), (abi.QMC_PREC_MIXED, "mixed")):
+            sm = hip.SingleMoves(wk, precision=prec)
+            sm.nacc.zero_()
+            ms = timed(lambda: sm.sweep(0.5), 2)
+            r[name] = {"sweep_ms": ms, "acceptance": float(sm.nacc.sum().item()) / (nw * d.nptot * 3),
+                       "electron_moves_per_s": 1e3 / ms * nw * d.nptot}
+            sm = None
+        r["eval_ms"] = timed(wk.start, 3)
+        res[tag] = r
+        wk = None
+        ctx.close()
     n = res["n122"]
     out = {
         "metric": "single-electron sweeps/s (N=122, 4096 walkers, FP64 inverse)",
