@@ -181,6 +181,17 @@ class QmcSemState(C.Structure):
     ]
 
 
+class QmcSemJob(C.Structure):
+    _fields_ = [
+        ("ctx", C.c_void_p),
+        ("w", C.c_void_p),
+        ("d", C.c_void_p),
+        ("j0", C.c_int32),
+        ("j1", C.c_int32),
+        ("sigma", C.c_double),
+    ]
+
+
 def limbs_to_state(limbs):
     """48-bit LCG state from the four 12-bit limbs savern writes."""
     l1, l2, l3, l4 = (int(v) for v in limbs)

@@ -235,10 +235,10 @@ struct SemWaves {
 };
 
 template <typename Z, int MR, int CG, bool U3>
-__global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves(DevSys s, int j0, int j1, int sp, double sigma,
-                                                      double* __restrict__ x, int64_t* __restrict__ rng,
-                                                      Z* __restrict__ ainv, int mm, double* __restrict__ dlogpsi,
-                                                      double* __restrict__ nacc) {
+__device__ __forceinline__ void sem_moves_walker(const DevSys& s, int w, int j0, int j1, int sp, double sigma,
+                                                 double* __restrict__ x, int64_t* __restrict__ rng,
+                                                 Z* __restrict__ ainv, int mm, double* __restrict__ dlogpsi,
+                                                 double* __restrict__ nacc) {
   typedef typename SemZ<Z>::R R;
   constexpr int T = MR * CG, EPT = MR / CG, NWV = T / 64 > 0 ? T / 64 : 1;
   constexpr int PMAX = (128 + T - 1) / T;  // partners per thread (N <= 128)
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves(
   extern __shared__ __attribute__((aligned(16))) unsigned char sem_lds[];
   double* xs = reinterpret_cast<double*>(sem_lds);  // positions [N][2]
   double* Gs = xs + 2 * s.nptot;                     // U3: G_p [N][2]
-  const int w = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int N = s.nptot, nmv = j1 - j0;
   const int m = pick2(s.np, sp), p0 = pick2(s.ipf, sp);
   const int l = tid / CG, cg = tid % CG;
@@ -452,6 +452,54 @@ __global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves(
   }
 }
 
+template <typename Z, int MR, int CG, bool U3>
+__global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves(const DevSys* __restrict__ sys,
+                                                                                 int j0, int j1, int sp,
+                                                                                 double sigma, double* x, int64_t* rng,
+                                                                                 Z* ainv, int mm, double* dlogpsi,
+                                                                                 double* nacc) {
+  sem_moves_walker<Z, MR, CG, U3>(*sys, blockIdx.x, j0, j1, sp, sigma, x, rng, ainv, mm, dlogpsi, nacc);
+}
+
+// Several jobs (contexts of different N, decks and walker sets) in one
+// launch: job k owns blocks w0[k] .. w0[k] + nw[k] - 1, one per walker; each
+// block reads its job's system description from device memory (uniform
+// loads) and runs the same per-walker body.  MR is the largest per-spin
+// count of the jobs; a job with fewer particles leaves the extra rows idle.
+struct SemJob {
+  const DevSys* sys;
+  double* x;
+  int64_t* rng;
+  void* ainv;
+  double* dlogpsi;
+  double* nacc;
+  double sigma;
+  int w0, nw, j0, j1, sp, mm;
+};
+#define SEM_MAX_PACK 16
+struct SemJobs {
+  SemJob job[SEM_MAX_PACK];
+  int njobs;
+};
+
+template <typename Z, int MR, int CG, bool U3>
+__global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves_packed(const SemJobs jobs) {
+  int k = 0;
+  while (k + 1 < jobs.njobs && (int)blockIdx.x >= jobs.job[k + 1].w0) k++;
+  const SemJob& jb = jobs.job[k];
+  sem_moves_walker<Z, MR, CG, U3>(*jb.sys, blockIdx.x - jb.w0, jb.j0, jb.j1, jb.sp, jb.sigma, jb.x, jb.rng,
+                                  (Z*)jb.ainv, jb.mm, jb.dlogpsi, jb.nacc);
+}
+
+// The context's system description in device memory, refreshed on the
+// stream before each launch that reads it (the g-vectors or the time step
+// may have changed since the last call).
+static int upload_sys(qmc_ctx* c, hipStream_t st) {
+  if (!c->d_sys) HIPCHK(hipMalloc(&c->d_sys, sizeof(DevSys)));
+  HIPCHK(hipMemcpyAsync(c->d_sys, &c->s, sizeof(DevSys), hipMemcpyHostToDevice, st));
+  return QMC_OK;
+}
+
 template <typename Z, int MR>
 static int launch_sem_moves(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, int j0, int j1, int sp,
                             double sigma, int mm, hipStream_t st) {
@@ -459,10 +507,10 @@ static int launch_sem_moves(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_stat
   const bool u3 = c->s.has_u3 != 0;
   const size_t lds = (u3 ? 4 : 2) * (size_t)c->s.nptot * sizeof(double);
   if (u3)
-    hipLaunchKernelGGL((k_sem_moves<Z, MR, CG, true>), dim3(w->nw), dim3(MR * CG), lds, st, c->s, j0, j1, sp,
+    hipLaunchKernelGGL((k_sem_moves<Z, MR, CG, true>), dim3(w->nw), dim3(MR * CG), lds, st, c->d_sys, j0, j1, sp,
                        sigma, w->x, w->rng, (Z*)d->ainv, mm, d->dlogpsi, d->nacc);
   else
-    hipLaunchKernelGGL((k_sem_moves<Z, MR, CG, false>), dim3(w->nw), dim3(MR * CG), lds, st, c->s, j0, j1, sp,
+    hipLaunchKernelGGL((k_sem_moves<Z, MR, CG, false>), dim3(w->nw), dim3(MR * CG), lds, st, c->d_sys, j0, j1, sp,
                        sigma, w->x, w->rng, (Z*)d->ainv, mm, d->dlogpsi, d->nacc);
   HIPCHK(hipGetLastError());
   return QMC_OK;
@@ -602,6 +650,8 @@ extern "C" int qmc_sem_moves(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_sta
   if (w->nw == 0 || j0 == j1) return QMC_OK;
   HIPCHK(hipSetDevice(c->device));
   if (!c->b_zero) return sem_moves_full(c, w, d, j0, j1, sigma, precision, (hipStream_t)stream);
+  rc = upload_sys(c, (hipStream_t)stream);
+  if (rc) return rc;
   const int mm = sem_mm(c);
   // one launch per spin the range touches, in order
   for (int sp = 0; sp < c->s.ntypes; sp++) {
@@ -609,6 +659,84 @@ extern "C" int qmc_sem_moves(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_sta
     if (a >= b) continue;
     rc = precision == QMC_PREC_MIXED ? sem_moves_spin<float2>(c, w, d, a, b, sp, sigma, mm, (hipStream_t)stream)
                                      : sem_moves_spin<double2>(c, w, d, a, b, sp, sigma, mm, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return QMC_OK;
+}
+
+template <typename Z, int MR, bool U3>
+static int launch_sem_packed(const SemJobs& jb, int nblocks, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((k_sem_moves_packed<Z, MR, 4, U3>), dim3(nblocks), dim3(MR * 4), lds, st, jb);
+  HIPCHK(hipGetLastError());
+  return QMC_OK;
+}
+
+template <typename Z>
+static int sem_packed_phase(const SemJobs& jb, int nblocks, int mr, bool u3, size_t lds, hipStream_t st) {
+  if (mr <= 16) return u3 ? launch_sem_packed<Z, 16, true>(jb, nblocks, lds, st) : launch_sem_packed<Z, 16, false>(jb, nblocks, lds, st);
+  if (mr <= 32) return u3 ? launch_sem_packed<Z, 32, true>(jb, nblocks, lds, st) : launch_sem_packed<Z, 32, false>(jb, nblocks, lds, st);
+  return u3 ? launch_sem_packed<Z, 64, true>(jb, nblocks, lds, st) : launch_sem_packed<Z, 64, false>(jb, nblocks, lds, st);
+}
+
+extern "C" int qmc_sem_moves_packed(const qmc_sem_job* jobs, int njobs, int precision, void* stream) {
+  if (!jobs || njobs < 0 || njobs > SEM_MAX_PACK) return arg_fail("between 0 and 16 jobs per packed call");
+  if (njobs == 0) return QMC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  for (int k = 0; k < njobs; k++) {
+    const qmc_sem_job& jb = jobs[k];
+    int rc = sem_check(jb.ctx, jb.w, jb.d, precision);
+    if (rc) return rc;
+    if (!jb.ctx->b_zero) return arg_fail("jobs with backflow cannot be packed (each move is a full evaluation)");
+    if (jb.ctx->device != jobs[0].ctx->device) return arg_fail("packed jobs must share one device");
+    if (jb.j0 < 0 || jb.j1 > jb.ctx->s.nptot || jb.j0 > jb.j1) return arg_fail("electron range out of bounds");
+    if (!(jb.sigma > 0.0)) return arg_fail("sigma must be positive");
+  }
+  HIPCHK(hipSetDevice(jobs[0].ctx->device));
+  for (int k = 0; k < njobs; k++) {
+    int rc = upload_sys(jobs[k].ctx, st);
+    if (rc) return rc;
+  }
+  // spin 0 of every job, then spin 1: each job sees its moves in the order
+  // qmc_sem_moves makes them.  Within a spin, one launch per row class
+  // (16, 32, 64 rows, as qmc_sem_moves picks), so that each job runs with
+  // the thread layout of its own call and its sums are added in the same
+  // order; the three-body variant also serves jobs without u3 (their u3
+  // terms are skipped and add nothing).
+  for (int sp = 0; sp < 2; sp++)
+  for (int cls = 0; cls < 3; cls++) {
+    SemJobs jb{};
+    int nblocks = 0, mr = 0;
+    bool u3 = false;
+    size_t lds = 0;
+    for (int k = 0; k < njobs; k++) {
+      qmc_ctx* c = jobs[k].ctx;
+      if (sp >= c->s.ntypes || jobs[k].w->nw == 0) continue;
+      const int m = c->s.np[sp];
+      if ((m <= 16 ? 0 : m <= 32 ? 1 : 2) != cls) continue;
+      const int a = std::max(jobs[k].j0, c->s.ipf[sp]), b = std::min(jobs[k].j1, c->s.ipf[sp] + c->s.np[sp]);
+      if (a >= b) continue;
+      SemJob& e = jb.job[jb.njobs++];
+      e.sys = c->d_sys;
+      e.x = jobs[k].w->x;
+      e.rng = jobs[k].w->rng;
+      e.ainv = jobs[k].d->ainv;
+      e.dlogpsi = jobs[k].d->dlogpsi;
+      e.nacc = jobs[k].d->nacc;
+      e.sigma = jobs[k].sigma;
+      e.w0 = nblocks;
+      e.nw = jobs[k].w->nw;
+      e.j0 = a;
+      e.j1 = b;
+      e.sp = sp;
+      e.mm = sem_mm(c);
+      nblocks += e.nw;
+      mr = std::max(mr, c->s.np[sp]);
+      u3 = u3 || c->s.has_u3;
+      lds = std::max(lds, (c->s.has_u3 ? 4 : 2) * (size_t)c->s.nptot * sizeof(double));
+    }
+    if (jb.njobs == 0) continue;
+    const int rc = precision == QMC_PREC_MIXED ? sem_packed_phase<float2>(jb, nblocks, mr, u3, lds, st)
+                                               : sem_packed_phase<double2>(jb, nblocks, mr, u3, lds, st);
     if (rc) return rc;
   }
   return QMC_OK;

@@ -59,6 +59,12 @@ module qmc_hip_mod
      type(c_ptr) :: ainv, dlogpsi, nacc
   end type qmc_sem_state
 
+  type, bind(C) :: qmc_sem_job
+     type(c_ptr) :: ctx, w, d
+     integer(c_int32_t) :: j0, j1
+     real(c_double) :: sigma
+  end type qmc_sem_job
+
   type, bind(C) :: qmc_rmc_params
      real(c_double) :: delta, etrial, mage, mage_r
      integer(c_int32_t) :: ntauskip, mdelta, rejection, wpiu, gstorto, nodalaction, ecut, bm_order
@@ -185,6 +191,14 @@ module qmc_hip_mod
        integer(c_int), value :: j0, j1, precision
        real(c_double), value :: sigma
      end function qmc_sem_moves
+
+     integer(c_int) function qmc_sem_moves_packed(jobs, njobs, precision, stream) &
+          bind(C, name='qmc_sem_moves_packed')
+       import :: c_int, c_ptr, qmc_sem_job
+       type(qmc_sem_job), intent(in) :: jobs(*)
+       integer(c_int), value :: njobs, precision
+       type(c_ptr), value :: stream
+     end function qmc_sem_moves_packed
 
      integer(c_int) function qmc_rmc_start(ctx, r, prm, stream) bind(C, name='qmc_rmc_start')
        import :: c_int, c_ptr, qmc_reptiles, qmc_rmc_params

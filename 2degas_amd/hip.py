@@ -19,7 +19,8 @@ EXPORTS = ("qmc_ctx_create", "qmc_ctx_destroy", "qmc_ctx_info", "qmc_set_twist",
            "qmc_last_error", "qmc_flops_per_eval", "qmc_timing", "qmc_timing_read",
            "qmc_vmc_start", "qmc_vmc_sweep", "qmc_vmc_status", "qmc_rmc_start",
            "qmc_rmc_sweep", "qmc_set_delta", "qmc_eval_batch_device_noalloc", "qmc_dmc_sweep",
-           "qmc_dmc_branch", "qmc_sem_refresh", "qmc_sem_moves")
+           "qmc_dmc_branch", "qmc_sem_refresh", "qmc_sem_moves",
+           "qmc_sem_moves_packed")
 
 _lib = None
 
@@ -78,6 +79,7 @@ def lib():
         L.qmc_sem_refresh.argtypes = [vp, C.POINTER(abi.QmcWalkers), C.POINTER(abi.QmcSemState), C.c_int, vp]
         L.qmc_sem_moves.argtypes = [vp, C.POINTER(abi.QmcWalkers), C.POINTER(abi.QmcSemState), C.c_int,
                                     C.c_int, C.c_double, C.c_int, vp]
+        L.qmc_sem_moves_packed.argtypes = [C.POINTER(abi.QmcSemJob), C.c_int, C.c_int, vp]
         _lib = L
     return _lib
 
@@ -401,6 +403,24 @@ class SingleMoves:
             drifts[int(k)] = worst
         ok = [k for k in sorted(drifts) if drifts[k] <= tol]
         return (ok[-1] if ok else min(drifts)), drifts
+
+
+def sem_moves_packed(moves, precision=0, stream=None):
+    """qmc_sem_moves_packed: moves is a list of (SingleMoves, j0, j1, sigma),
+    one per job (up to 16), all with the given precision."""
+    import torch
+    jobs = (abi.QmcSemJob * max(1, len(moves)))()
+    for k, (sm, j0, j1, sigma) in enumerate(moves):
+        if sm.precision != int(precision):
+            raise ValueError("every packed job must use the call's precision")
+        jobs[k].ctx = sm.wk.ctx.h.value
+        jobs[k].w = C.cast(C.pointer(sm.wk.s), C.c_void_p).value
+        jobs[k].d = C.cast(C.pointer(sm.s), C.c_void_p).value
+        jobs[k].j0, jobs[k].j1, jobs[k].sigma = int(j0), int(j1), float(sigma)
+    st = stream if stream is not None else torch.cuda.current_stream()
+    rc = lib().qmc_sem_moves_packed(jobs, len(moves), int(precision), C.c_void_p(st.cuda_stream))
+    if rc:
+        raise QmcError(rc, "qmc_sem_moves_packed")
 
 
 # fixed fields of one reptation trace row before the per-bead probabilities

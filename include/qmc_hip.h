@@ -425,6 +425,24 @@ int qmc_sem_moves(qmc_ctx* ctx, const qmc_walkers* w, const qmc_sem_state* d, in
                   double sigma, int precision, void* stream);
 
 /*
+ * qmc_sem_moves for up to 16 jobs (contexts of any N, decks and walker sets
+ * on one device, none with backflow) in one launch per spin: spin 0's moves
+ * of every job, then spin 1's.  Each job's results are those of its own
+ * qmc_sem_moves call with the same range, bit for bit.  No counterpart in
+ * the reference (one system per run); for services that run many small
+ * jobs at once.
+ */
+typedef struct qmc_sem_job {
+  qmc_ctx* ctx;
+  const qmc_walkers* w;
+  const qmc_sem_state* d;
+  int32_t j0, j1;    /* electrons j0..j1-1 of every walker of the job        */
+  double sigma;      /* Gaussian step per dimension                           */
+} qmc_sem_job;
+
+int qmc_sem_moves_packed(const qmc_sem_job* jobs, int njobs, int precision, void* stream);
+
+/*
  * Per-kernel timing with HIP events recorded on the launch stream around each
  * kernel of every evaluation while enabled (no counterpart in the reference,
  * which has no timers).  qmc_timing(ctx, 1) resets and enables,

@@ -93,7 +93,8 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
     structs = {"qmc_tables": abi.QmcTables, "qmc_info": abi.QmcInfo,
                "qmc_walkers": abi.QmcWalkers, "qmc_reptiles": abi.QmcReptiles,
                "qmc_rmc_params": abi.QmcRmcParams, "qmc_dmc_params": abi.QmcDmcParams,
-               "qmc_dmc_state": abi.QmcDmcState, "qmc_sem_state": abi.QmcSemState}
+               "qmc_dmc_state": abi.QmcDmcState, "qmc_sem_state": abi.QmcSemState,
+               "qmc_sem_job": abi.QmcSemJob}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "qmc_hip.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -133,7 +134,8 @@ def test_fortran_bind_c_types_match_the_c_layout(tmp_path):
     structs = {"qmc_tables": abi.QmcTables, "qmc_info": abi.QmcInfo,
                "qmc_walkers": abi.QmcWalkers, "qmc_reptiles": abi.QmcReptiles,
                "qmc_rmc_params": abi.QmcRmcParams, "qmc_dmc_params": abi.QmcDmcParams,
-               "qmc_dmc_state": abi.QmcDmcState, "qmc_sem_state": abi.QmcSemState}
+               "qmc_dmc_state": abi.QmcDmcState, "qmc_sem_state": abi.QmcSemState,
+               "qmc_sem_job": abi.QmcSemJob}
     lines = ["program probe", "  use iso_c_binding", "  use qmc_hip_mod", "  implicit none"]
     for cname in structs:
         lines.append(f"  type({cname}), target :: v_{cname}")

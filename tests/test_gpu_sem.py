@@ -192,6 +192,49 @@ def test_refresh_tuner_measures_drift():
     assert drifts[256] < 1e-2
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_packed_jobs_equal_separate_calls(tmp_path, precision):
+    """Four jobs of different N, decks and walker counts (c26; a generated
+    N=26 deck with u3; N=58 with u3; N=122) moved by packed launches (one per
+    spin and row class) give bit for bit what each job's own qmc_sem_moves
+    calls give: positions, streams, inverses, log|psi| sums and counts."""
+    print(f"\n[gpu] sem packed jobs vs separate calls, precision {precision}", flush=True)
+    import torch
+    hip = pkg("hip")
+    dg = pkg("deckgen")
+    decks = [load_deck("c26"), _u3_deck(tmp_path / "a")]
+    for tag, nu, sh in (("b", 29, 12), ("c", 61, 12)):
+        dg.make_deck(str(tmp_path / tag), "pk", nu, nu, 5.0, 5, nshell_k=sh, u3=tag == "b", backflow="zero",
+                     jastrow="pade")
+        decks.append(pkg("deck").read_deck(str(tmp_path / tag)))
+    sizes, sigmas = [24, 40, 16, 8], [0.5, 0.4, 0.6, 0.5]
+
+    def make():
+        out = []
+        for k, d in enumerate(decks):
+            ctx = hip.Context(d, twist=(1, 1))
+            x = dg.lattice_configs(sizes[k], d.nptot, d.el[0], 30 + k)
+            wk = hip.Walkers(ctx, x, _states(sizes[k], 500 + 40 * k))
+            wk.start()
+            sm = hip.SingleMoves(wk, precision=precision)
+            sm.refresh()
+            out.append(sm)
+        return out
+
+    sep, pak = make(), make()
+    for _ in range(2):
+        for k, sm in enumerate(sep):
+            sm.moves(0, decks[k].nptot, sigmas[k])
+        hip.sem_moves_packed([(sm, 0, decks[k].nptot, sigmas[k]) for k, sm in enumerate(pak)], precision)
+    torch.cuda.synchronize()
+    for k in range(len(decks)):
+        a = [t.cpu().numpy() for t in (sep[k].wk.x, sep[k].wk.rng, sep[k].ainv, sep[k].dlogpsi, sep[k].nacc)]
+        b = [t.cpu().numpy() for t in (pak[k].wk.x, pak[k].wk.rng, pak[k].ainv, pak[k].dlogpsi, pak[k].nacc)]
+        print(f"   job {k} N={decks[k].nptot} walkers {sizes[k]} accepted {a[4].sum():.0f}", flush=True)
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)
+
+
 def test_moves_range_checks():
     """Electron ranges out of bounds and a step that is not positive are
     refused."""
