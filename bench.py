@@ -553,6 +553,29 @@ def bench_sem(args, hip, deck, dg):
         res[tag] = r
         wk = None
         ctx.close()
+    # packing: 16 small jobs (N 26, 42, 58, 74, four each; 256 walkers; u2
+    # only, zero backflow), one sweep each as separate calls on one stream
+    # against one packed call per spin and row class
+    jobs = []
+    for k in range(16):
+        nu = (13, 21, 29, 37)[k % 4]
+        tmp = tempfile.mkdtemp(prefix="degas_sem_")
+        deckdir = os.path.join(tmp, "deck")
+        dg.make_deck(deckdir, "pk", nu, nu, 2.0 + k % 5, 12, u3=False, backflow="zero", jastrow="pade")
+        d = deck.read_deck(deckdir)
+        ctx = hip.Context(d, twist=(1, 1), device=0)
+        x = dg.lattice_configs(256, d.nptot, d.el[0], 700 + k)
+        sts = [abi.limbs_to_state([3535, 3467, 3262, (2 * (1549 + q + 256 * k) + 1) % 4096]) for q in range(256)]
+        wk = hip.Walkers(ctx, x, sts)
+        wk.start()
+        sm = hip.SingleMoves(wk)
+        sm.refresh()
+        jobs.append((sm, 0, d.nptot, 0.5))
+    sep_ms = timed(lambda: [sm.moves(a, b, sg) for sm, a, b, sg in jobs], 10)
+    pak_ms = timed(lambda: hip.sem_moves_packed(jobs), 10)
+    res["packed16"] = {"jobs": 16, "walkers_per_job": 256, "N": [13 * 2, 21 * 2, 29 * 2, 37 * 2],
+                       "separate_ms": sep_ms, "packed_ms": pak_ms, "speedup": sep_ms / pak_ms}
+    jobs = None
     n = res["n122"]
     out = {
         "metric": "single-electron sweeps/s (N=122, 4096 walkers, FP64 inverse)",
