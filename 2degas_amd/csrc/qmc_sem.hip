@@ -495,7 +495,6 @@ __global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves_
 // stream before each launch that reads it (the g-vectors or the time step
 // may have changed since the last call).
 static int upload_sys(qmc_ctx* c, hipStream_t st) {
-  if (!c->d_sys) HIPCHK(hipMalloc(&c->d_sys, sizeof(DevSys)));
   HIPCHK(hipMemcpyAsync(c->d_sys, &c->s, sizeof(DevSys), hipMemcpyHostToDevice, st));
   return QMC_OK;
 }

@@ -59,6 +59,9 @@ class JobSpec:
     # mage, metropolis_test): walkers started next to a node, where every
     # proposal is rejected, move again instead of freezing their energy
     mage: float = 50.0
+    # single-electron moves (serve(moves="single")): the Gaussian step per
+    # dimension, in units of rs; a sweep moves every electron once
+    sigma: float = 0.5
     arrival: float = 0.0       # seconds after the service starts
     seed_limbs: tuple = (3535, 3467, 3262, 1549)
 
@@ -82,14 +85,23 @@ class JobResult:
         return self.done - self.arrival
 
 
-def job_bytes(ctx, nw):
+def single_bytes(ctx, nw):
+    """Bytes of a SingleMoves state for nw walkers (FP64 inverses of the
+    larger spin, the log|psi| and count accumulators) and of the context's
+    device copy of its system description."""
+    mm = max(int(v) for v in ctx.deck.np[:ctx.deck.ntypes])
+    return nw * (2 * mm * mm * 16 + 2 * 8)
+
+
+def job_bytes(ctx, nw, single=False):
     """Device memory of a VMC job of nw walkers on context ctx: what the
     context holds (tables, g-vectors), its evaluation workspace and VMC
     proposal buffers for nw walkers (per-walker sizes from qmc_ctx_info) and
-    the walker state tensors."""
+    the walker state tensors; with single-electron moves, their state."""
     m = ctx.device_bytes()
     return (m["dev_bytes"] + nw * (m["eval_bytes_per_walker"] + m["vmc_bytes_per_walker"])
-            + hip.Walkers.state_bytes(nw, ctx.deck.nptot, ctx.info["n_props"]))
+            + hip.Walkers.state_bytes(nw, ctx.deck.nptot, ctx.info["n_props"])
+            + (single_bytes(ctx, nw) if single else 0))
 
 
 def block_error(blocks):
@@ -107,8 +119,11 @@ class _Running:
     """A job: its deck and context from construction (the tables on the
     device, the memory it will need known), its walkers from start()."""
 
-    def __init__(self, spec, workdir):
+    def __init__(self, spec, workdir, single=False):
         self.spec = spec
+        self.single = single
+        if single and spec.backflow != "zero":
+            raise ValueError(f"job {spec.job_id}: single-electron moves are packed only without backflow")
         ddir = os.path.join(workdir, f"job{spec.job_id}")
         deckgen.make_deck(ddir, "job", spec.nup, spec.ndown, spec.rs, spec.shells,
                           jastrow=spec.jastrow, u3=spec.u3, backflow=spec.backflow,
@@ -116,7 +131,7 @@ class _Running:
         self.deck = deck.read_deck(ddir)
         self.ctx = hip.Context(self.deck, twist=(1, 1))
         self.jetot = self.ctx.info["jetot"] - 1
-        self.need = job_bytes(self.ctx, spec.nw)
+        self.need = job_bytes(self.ctx, spec.nw, single)
 
     def start(self, t_admit):
         import torch
@@ -128,12 +143,18 @@ class _Running:
         # walker k draws the stream of MPI rank k (reptation.f:37-39)
         states = [abi.limbs_to_state([s[0], s[1], s[2], (2 * (s[3] + k) + 1) % 4096])
                   for k in range(spec.nw)]
-        self.stream = torch.cuda.Stream()
+        # all-electron jobs run on streams of their own; single-electron
+        # jobs share the service's stream, where their moves are packed
+        self.stream = torch.cuda.current_stream() if self.single else torch.cuda.Stream()
         self.delta = spec.delta if spec.delta > 0 else spec.rs * spec.rs / self.deck.nptot
         with torch.cuda.stream(self.stream):
             self.walkers = hip.Walkers(self.ctx, x, states)
             self.walkers.start(stream=self.stream, delta=self.delta)
+            if self.single:
+                self.sm = hip.SingleMoves(self.walkers)
+                self.sm.refresh(stream=self.stream)
         self.nblk = 0
+        self.nsweep = 0
         self.blocks = []
         self.evals = 0
         self.event = torch.cuda.Event()
@@ -170,21 +191,35 @@ class _Running:
         err = block_error(self.blocks)
         held = self.ctx.device_bytes()["dev_bytes"] + hip.Walkers.state_bytes(
             self.spec.nw, self.deck.nptot, self.ctx.info["n_props"])
+        if self.single:
+            held += self.sm.ainv.numel() * self.sm.ainv.element_size() + 2 * 8 * self.spec.nw
         return JobResult(self.spec.job_id, float(np.mean(self.blocks)) if self.blocks else math.nan,
                          err, list(self.blocks), self.spec.arrival, self.t_admit, t_done,
                          self.evals, err <= self.spec.target, self.need, held)
 
     def close(self):
         self.stream.synchronize()
+        self.sm = None
         self.walkers = None
         self.ctx.close()
 
 
-def serve(specs, mem_budget=64 << 30, clock=None, max_active=64, poll_s=0.0005):
+def serve(specs, mem_budget=64 << 30, clock=None, max_active=64, poll_s=0.0005, moves="all"):
     """Run the jobs to their error targets with continuous batching.
     Returns (results by job id, summary dict).  The latency percentiles are
     over the jobs that reached their target; `capped` counts the jobs that
-    stopped at max_blocks without it."""
+    stopped at max_blocks without it.
+
+    moves="all": each job runs the reference's all-electron VMC step on a
+    stream of its own.  moves="single" (jobs without backflow): every
+    electron of every running job is moved once per sweep by packed
+    single-electron launches (up to 16 jobs of any N per launch), then each
+    job's walkers are evaluated for their local energies; a block is
+    steps_per_block sweeps, and jobs are admitted between sweeps."""
+    if moves == "single":
+        return _serve_single(specs, mem_budget, clock, max_active)
+    if moves != "all":
+        raise ValueError(f"unknown moves {moves!r}")
     import torch
     clock = clock or time.perf_counter
     pending = sorted(specs, key=lambda s: (s.arrival, s.job_id))
@@ -231,7 +266,65 @@ def serve(specs, mem_budget=64 << 30, clock=None, max_active=64, poll_s=0.0005):
         if not progressed:
             time.sleep(poll_s)
     torch.cuda.synchronize()
-    wall = clock() - t0
+    return results, _summary(results, clock() - t0, blocks_run)
+
+
+def _serve_single(specs, mem_budget, clock, max_active):
+    import torch
+    clock = clock or time.perf_counter
+    pending = sorted(specs, key=lambda s: (s.arrival, s.job_id))
+    workdir = tempfile.mkdtemp(prefix="degas_serve_")
+    active, results = [], {}
+    used = 0
+    head = None
+    t0 = clock()
+    blocks_run = 0
+    while pending or active:
+        now = clock() - t0
+        while pending and pending[0].arrival <= now and len(active) < max_active:
+            if head is None:
+                head = _Running(pending[0], workdir, single=True)
+            if active and used + head.need > mem_budget:
+                break
+            pending.pop(0)
+            r, head = head, None
+            r.start(clock() - t0)
+            r.walkers.blk.zero_()
+            used += r.need
+            active.append(r)
+        if not active:
+            time.sleep(max(0.0, min(0.05, pending[0].arrival - now)))
+            continue
+        # one sweep of every running job, 16 jobs per packed call
+        for g in range(0, len(active), 16):
+            hip.sem_moves_packed([(r.sm, 0, r.deck.nptot, r.spec.sigma) for r in active[g:g + 16]])
+        for r in active:
+            r.walkers.start()
+            r.walkers.blk += r.walkers.props
+            r.nsweep += 1
+        still = []
+        for r in active:
+            if r.nsweep < r.spec.steps_per_block:
+                still.append(r)
+                continue
+            r.nsweep = 0
+            blocks_run += 1
+            r.event.record()
+            if r.finish_block():
+                res = r.result(clock() - t0)
+                results[res.job_id] = res
+                used -= r.need
+                r.close()
+            else:
+                r.walkers.blk.zero_()
+                r.sm.refresh()
+                still.append(r)
+        active = still
+    torch.cuda.synchronize()
+    return results, _summary(results, clock() - t0, blocks_run)
+
+
+def _summary(results, wall, blocks_run):
     lat = np.array([r.latency for r in results.values() if r.converged])
     evals = sum(r.evals for r in results.values())
     summary = {
@@ -245,12 +338,12 @@ def serve(specs, mem_budget=64 << 30, clock=None, max_active=64, poll_s=0.0005):
         "blocks": blocks_run,
         "converged": int(sum(r.converged for r in results.values())),
     }
-    return results, summary
+    return summary
 
 
 def poisson_jobs(n, rate, seed, nw=256, target=2e-4, sizes=((13, 13), (29, 29), (37, 37), (61, 61),
                                                            (37, 0), (49, 0), (61, 0)),
-                 rs_range=(1.0, 20.0), max_blocks=400, steps_per_block=20):
+                 rs_range=(1.0, 20.0), max_blocks=400, steps_per_block=20, backflow="yk"):
     """n job specs with Poisson arrivals at `rate` per second, N drawn from
     `sizes` (paramagnetic and polarized), rs uniform in rs_range, and the
     wavefunction parameters perturbed by 10% from a per-job seed."""
@@ -263,5 +356,6 @@ def poisson_jobs(n, rate, seed, nw=256, target=2e-4, sizes=((13, 13), (29, 29), 
         rs = float(rng.uniform(*rs_range))
         out.append(JobSpec(job_id=j, nup=int(nu), ndown=int(nd), rs=rs,
                            param_seed=int(rng.integers(1, 1 << 30)), nw=nw, target=target,
-                           arrival=t, max_blocks=max_blocks, steps_per_block=steps_per_block))
+                           arrival=t, max_blocks=max_blocks, steps_per_block=steps_per_block,
+                           backflow=backflow))
     return out

@@ -149,6 +149,10 @@ def main():
                     help="single-electron moves with Sherman-Morrison updates instead: plain "
                          "Slater-Jastrow N=122 (61+61, rs=10, Pade u2, zero backflow, 200 k-shells) "
                          "with 4096 walkers and N=26 with 256, FP64 and mixed inverses")
+    ap.add_argument("--serve-moves", choices=("all", "single", "compare"), default="all",
+                    help="--serve: the reference's all-electron moves (jobs with the n26 "
+                         "backflow), packed single-electron moves (jobs without backflow), or "
+                         "both on the same backflow-free job mix")
     ap.add_argument("--jobs", type=int, default=40)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--max-blocks", type=int, default=1000,
@@ -602,6 +606,8 @@ def bench_serve(args):
     """Continuous batching of independent VMC jobs: p50/p99 time from arrival
     to the requested error bar, jobs/s and aggregate local-energy evals/s."""
     sv = importlib.import_module("2degas_amd.serve")
+    if args.serve_moves != "all":
+        return bench_serve_single(args, sv)
     specs = sv.poisson_jobs(args.jobs, args.rate, seed=2024, nw=256, target=2e-4,
                             max_blocks=args.max_blocks, steps_per_block=20)
     results, summary = sv.serve(specs)
@@ -630,6 +636,38 @@ def bench_serve(args):
                      "blocks": len(r.blocks), "error": r.error, "converged": r.converged,
                      "latency_s": r.latency, "device_bytes": r.device_bytes}
                     for s in specs for r in [results[s.job_id]]],
+    }
+    print(json.dumps(out))
+
+
+def bench_serve_single(args, sv):
+    """The served workload with jobs without backflow (u2 only): packed
+    single-electron sweeps (20-sweep blocks, every electron moved once per
+    sweep, the local energy evaluated after each sweep), and with
+    --serve-moves compare the all-electron service on the same jobs."""
+    modes = ("single", "all") if args.serve_moves == "compare" else ("single",)
+    legs = {}
+    for mode in modes:
+        specs = sv.poisson_jobs(args.jobs, args.rate, seed=2024, nw=256, target=2e-4,
+                                max_blocks=args.max_blocks, steps_per_block=20, backflow="zero")
+        results, summary = sv.serve(specs, moves=mode)
+        legs[mode] = dict(summary, per_job=[{"job": r.job_id, "N": s.nup + s.ndown, "rs": round(s.rs, 3),
+                                             "blocks": len(r.blocks), "converged": r.converged,
+                                             "latency_s": r.latency}
+                                            for s in specs for r in [results[s.job_id]]])
+    lead = legs["single"]
+    out = {
+        "metric": "served VMC jobs (no backflow), packed single-electron moves: p50/p99 time to 1e-4 Ha/electron",
+        "value": lead["p99_latency_s"],
+        "unit": "s (p99)",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.jobs} VMC jobs, Poisson {args.rate}/s, N in 26/58/74/122 "
+                               "or polarized 37/49/61, rs U[1,20], Pade u2, zero backflow, 256 walkers, "
+                               "20-sweep blocks", "jobs": args.jobs, "rate": args.rate},
+        "legs": legs,
     }
     print(json.dumps(out))
 

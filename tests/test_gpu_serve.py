@@ -60,3 +60,37 @@ def test_admission_counts_what_each_job_allocates():
         assert b.admitted >= a.done
     assert summary["capped"] == 0 and summary["converged"] == 3
     assert summary["p99_latency_s"] >= summary["p50_latency_s"] > 0
+
+
+def _single_specs(sv):
+    common = dict(nw=64, steps_per_block=4, target=0.0, min_blocks=4, max_blocks=5, shells=10,
+                  backflow="zero", u3=True)
+    return [sv.JobSpec(0, 13, 13, 5.0, param_seed=21, arrival=0.0, **common),
+            sv.JobSpec(1, 29, 29, 2.0, param_seed=22, arrival=0.02, **common),
+            sv.JobSpec(2, 37, 0, 8.0, param_seed=23, arrival=0.04, sigma=0.4, **common)]
+
+
+def test_single_electron_service_alone_and_in_the_mix():
+    """moves="single": jobs without backflow share packed single-electron
+    launches; each job's block means are the same alone and in the mix, and
+    its admission bytes (with the single-electron state) equal what it held
+    when it retired."""
+    print("\n[gpu] serve single-electron moves, alone vs mix", flush=True)
+    sv = pkg("serve")
+    alone = {}
+    for s in _single_specs(sv):
+        s.arrival = 0.0
+        res, _ = sv.serve([s], moves="single")
+        alone[s.job_id] = res[s.job_id].blocks
+    mixed, summary = sv.serve(_single_specs(sv), moves="single")
+    print(f"   summary {summary}", flush=True)
+    assert summary["jobs"] == 3
+    for j, blocks in alone.items():
+        assert len(blocks) == 5
+        assert np.all(np.isfinite(blocks))
+        assert mixed[j].blocks == blocks, f"job {j} differs in the mix"
+        assert mixed[j].predicted_bytes == mixed[j].device_bytes > 0
+    with pytest.raises(ValueError):
+        s = _single_specs(sv)[0]
+        s.backflow = "yk"
+        sv.serve([s], moves="single")
