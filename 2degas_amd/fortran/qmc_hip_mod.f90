@@ -200,6 +200,14 @@ module qmc_hip_mod
        type(c_ptr), value :: stream
      end function qmc_sem_moves_packed
 
+     integer(c_int) function qmc_autotune(ctx, nw, kchunk_out, ms_out, stream) bind(C, name='qmc_autotune')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, stream
+       integer(c_int), value :: nw
+       integer(c_int32_t), intent(out) :: kchunk_out(2)
+       real(c_double), intent(out) :: ms_out(2)
+     end function qmc_autotune
+
      integer(c_int) function qmc_rmc_start(ctx, r, prm, stream) bind(C, name='qmc_rmc_start')
        import :: c_int, c_ptr, qmc_reptiles, qmc_rmc_params
        type(c_ptr), value :: ctx, stream

@@ -20,7 +20,7 @@ EXPORTS = ("qmc_ctx_create", "qmc_ctx_destroy", "qmc_ctx_info", "qmc_set_twist",
            "qmc_vmc_start", "qmc_vmc_sweep", "qmc_vmc_status", "qmc_rmc_start",
            "qmc_rmc_sweep", "qmc_set_delta", "qmc_eval_batch_device_noalloc", "qmc_dmc_sweep",
            "qmc_dmc_branch", "qmc_sem_refresh", "qmc_sem_moves",
-           "qmc_sem_moves_packed")
+           "qmc_sem_moves_packed", "qmc_autotune")
 
 _lib = None
 
@@ -80,6 +80,7 @@ def lib():
         L.qmc_sem_moves.argtypes = [vp, C.POINTER(abi.QmcWalkers), C.POINTER(abi.QmcSemState), C.c_int,
                                     C.c_int, C.c_double, C.c_int, vp]
         L.qmc_sem_moves_packed.argtypes = [C.POINTER(abi.QmcSemJob), C.c_int, C.c_int, vp]
+        L.qmc_autotune.argtypes = [vp, C.c_int, ip, dp, vp]
         _lib = L
     return _lib
 
@@ -134,6 +135,19 @@ class Context:
         rc = lib().qmc_set_twist(self.h, abi.dptr(self.gvec), self.gvec.shape[0])
         if rc:
             raise QmcError(rc, "qmc_set_twist")
+
+    def autotune(self, nw=1024, stream=None):
+        """qmc_autotune: the pair kernels' phase chunk picked by timing nw
+        synthetic configurations; returns ((kchunk FP64, kchunk mixed),
+        (ms FP64, ms mixed))."""
+        import torch
+        kc = np.zeros(2, np.int32)
+        ms = np.zeros(2)
+        st = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib().qmc_autotune(self.h, int(nw), abi.iptr(kc), abi.dptr(ms), C.c_void_p(st.cuda_stream))
+        if rc:
+            raise QmcError(rc, "qmc_autotune")
+        return (int(kc[0]), int(kc[1])), (float(ms[0]), float(ms[1]))
 
     def set_delta(self, delta):
         """Time step read by the drift cutoff (decks with adrift != 0)."""

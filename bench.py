@@ -128,6 +128,8 @@ def main():
     ap.add_argument("--rs", type=float, default=10.0)
     ap.add_argument("--shells", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the size-based phase chunk of the pair kernels instead of qmc_autotune's")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--rqmc", action="store_true",
                     help="reptation workload instead: N=58 (29+29) rs=5 backflow, 512 reptiles "
@@ -195,6 +197,10 @@ def main():
     N = d.nptot
     x = dg.lattice_configs(args.nw, N, d.el[0], 12345)
     ctx = hip.Context(d, twist=(1, 1), device=0)
+    tune = None
+    if not args.no_autotune:
+        (k64, kmix), (ms64, msmix) = ctx.autotune(args.nw)
+        tune = {"kchunk_fp64": k64, "kchunk_mixed": kmix, "tuning_ms_fp64": ms64, "tuning_ms_mixed": msmix}
     dev = torch.device("cuda:0")
     xt = torch.tensor(x, dtype=torch.float64, device=dev)
     props = torch.zeros((args.nw, ctx.info["n_props"]), dtype=torch.float64, device=dev)
@@ -259,6 +265,7 @@ def main():
         "eval_tflops_formula": total_flops * value / 1e12,
         "guard_fired": guard,
         "mixed": mixed,
+        "autotune": tune,
     }
     if not args.no_cpu:
         cb = cpu_baseline(deckdir, x, args.cpu_seconds)

@@ -455,6 +455,20 @@ int qmc_sem_moves_packed(const qmc_sem_job* jobs, int njobs, int precision, void
 int qmc_timing(qmc_ctx* ctx, int enable);
 int qmc_timing_read(qmc_ctx* ctx, double* ms, int* nlaunch);
 
+/*
+ * Autotuner of the pair kernels' phase chunk (no counterpart in the
+ * reference): times full evaluations of nw synthetic configurations
+ * (uniform in the box) on `stream` for the size-based default chunk and
+ * chunks of 1, 2, 4, ... 32 particles (those whose LDS fits in 64 KB), for
+ * the FP64 and the mixed-precision
+ * pair kernel, and keeps the fastest for the context.  kchunk_out[2] and
+ * ms_out[2] (optional) receive the chosen chunks and their ms per
+ * evaluation.  Different chunks add the rho_k phases in a different order
+ * (rounding only).  Decks whose k-vectors do not factor into distinct
+ * components have nothing to tune (kchunk_out 1).
+ */
+int qmc_autotune(qmc_ctx* ctx, int nw, int32_t* kchunk_out, double* ms_out, void* stream);
+
 /* Human-readable text of an error code, and of the last HIP error seen. */
 const char* qmc_strerror(int code);
 const char* qmc_last_error(void);

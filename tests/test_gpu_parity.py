@@ -133,3 +133,21 @@ def test_rejects_unsupported_deck_before_any_launch():
     d.iu3[2] = 0
     with pytest.raises(hip.QmcError):
         hip.Context(d)
+
+
+@pytest.mark.parametrize("fn", ["eval_g122_t11.npz", "eval_g58_t11.npz"])
+def test_autotuned_phase_chunk_keeps_parity(fn):
+    """qmc_autotune picks the pair kernels' phase chunk by timing; the
+    chosen chunk is one of its candidates and the context still matches the
+    reference's goldens to 1e-10 (the chunk changes the order in which the
+    rho_k phases are added, rounding only)."""
+    print(f"\n[gpu] autotuned chunk, golden {fn}", flush=True)
+    hip = pkg("hip")
+    gold = load_golden(fn)
+    c = hip.Context(load_deck(fn.split("_")[1]), twist=tuple(gold["twist"]))
+    (k64, kmix), (ms64, msmix) = c.autotune(256)
+    print(f"   kchunk FP64 {k64} ({ms64:.3f} ms), mixed {kmix} ({msmix:.3f} ms)", flush=True)
+    assert k64 >= 1 and kmix >= 1 and ms64 > 0 and msmix > 0
+    out = c.eval(gold["x"])
+    errs = compare_eval(out, gold, gold["scalars"].shape[1], RTOL, fn)
+    print("   max err", max(errs.values()), flush=True)
