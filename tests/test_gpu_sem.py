@@ -235,6 +235,40 @@ def test_packed_jobs_equal_separate_calls(tmp_path, precision):
             assert np.array_equal(u, v)
 
 
+@pytest.mark.parametrize("nup,ndown", [(1, 1), (2, 0), (15, 16), (16, 17), (32, 31), (33, 1), (64, 0),
+                                       (64, 63)])
+def test_sizes_across_row_classes(tmp_path, nup, ndown):
+    """Spins of 1 to 64 particles, on both sides of the 16- and 32-row
+    class edges and polarized: after two sweeps the updated inverses equal
+    fresh ones and the summed log|psi| change equals the full
+    evaluations'."""
+    print(f"\n[gpu] sem sizes {nup}+{ndown}", flush=True)
+    import torch
+    dg = pkg("deckgen")
+    n = nup + ndown
+    shells = 5
+    L = float(np.sqrt(n * np.pi))
+    while 2 * len(dg.klist(L, shells)) + 1 < max(nup, ndown):
+        shells += 2
+    dg.make_deck(str(tmp_path), "sz", nup, ndown, 5.0, 5, nshell_k=shells, u3=True, backflow="zero",
+                 jastrow="pade")
+    d = pkg("deck").read_deck(str(tmp_path))
+    d, ctx, wk, sm = _setup(None, 8, deck=d)
+    jltf = ctx.info["jltf"] - 1
+    ltf0 = wk.props[:, jltf].clone()
+    sm.refresh()
+    for _ in range(2):
+        sm.sweep(0.5)
+    worst = _check_inverse_exact(d, ctx, wk, sm, 1e-8)
+    wk.start()
+    torch.cuda.synchronize()
+    err = float(np.max(np.abs(sm.dlogpsi.cpu().numpy() + (wk.props[:, jltf] - ltf0).cpu().numpy())))
+    acc = float(sm.nacc.sum().item()) / (8 * 2 * n)
+    print(f"   inverse rel err {worst:.1e}  |dlogpsi - full| {err:.1e}  acceptance {acc:.3f}", flush=True)
+    assert acc > 0.05
+    assert err < 1e-8
+
+
 def test_moves_range_checks():
     """Electron ranges out of bounds and a step that is not positive are
     refused."""
