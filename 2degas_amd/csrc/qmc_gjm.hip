@@ -138,7 +138,7 @@ __device__ __forceinline__ void gm_pivot_steps(GmShared& sh, zd (&P)[4], zd (&Fp
   }
 }
 
-template <bool PART>
+template <bool PART, bool VAL>
 __device__ __forceinline__ void gjm_run(DevSys s, WsLayout L, double* __restrict__ ws,
                                         double* __restrict__ Fbuf) {
   extern __shared__ __attribute__((aligned(16))) unsigned char gm_lds[];
@@ -266,112 +266,114 @@ __device__ __forceinline__ void gjm_run(DevSys s, WsLayout L, double* __restrict
     __syncthreads();
   }
 
-  // ---- phase 2
-  const int j = col;
-  const double qx = sh.q[2 * j], qy = sh.q[2 * j + 1];
-  double* Fd = ws + (int64_t)w * L.per_walker + pick2(L.off_det, sp) + DS_N + 2 * s.nptot + 3 * m;
-  // H_b = S G'_b for output column tile ct (particles j), row tiles 2 rh + t:
-  // h[t][b][0] real, h[t][b][1] imaginary.  Beside it the wavefronts of row
-  // half 0 form diag(F_a F_b)_j = diag(A^-1 G_a G_b A)_j
-  // = sum_k' S(p_j, k') (-g^a g^b)_{p_k'} A(p_k', j) / piv_j from the same
-  // A(p_k', j) values (lane (lr, lc) summing k' = lr mod 4 for particle j, the
-  // four partial sums combined after the loop), O(m^2) in all.  The sin/cos
-  // constants stay in scalar registers here, where the accumulators take the
-  // vector ones.
-  const bool dg = rh == 0;
-  int prj = 0, swj = 0;
-  zd pij = zmk(0.0, 0.0);
-  if (dg && j < m) {
-    prj = sh.prow[j];
-    swj = gm_swz(prj);
-    pij = sh.pinv[j];
-  }
-  double dxx = 0.0, dxy = 0.0, dyy = 0.0;
-  gm_d4 h[2][2][2];
-#pragma unroll
-  for (int t = 0; t < 2; t++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) h[t][b][0] = h[t][b][1] = gm_d4{0.0, 0.0, 0.0, 0.0};
-  // a wavefront with no live tile has neither outputs nor diagonal terms
-  // (its particles are >= m)
-  for (int ks = 0; live1 && 4 * ks < m; ks++) {
-    const int kp = 4 * ks + lr;
-    double bx_re = 0.0, bx_im = 0.0, by_re = 0.0, by_im = 0.0;
-    if (kp < m && j < m) {
-      const int o = sh.prow[kp];
-      const double gx = sh.g[2 * o], gy = sh.g[2 * o + 1];
-      double sv, cv;
-      qsincos_sreg(gx * qx + gy * qy, &sv, &cv);
-      bx_re = -gx * sv;
-      bx_im = gx * cv;
-      by_re = -gy * sv;
-      by_im = gy * cv;
-      if (dg) {
-        const int ad = prj * 64 + (kp ^ swj);
-        const double wr = zmul(zmul(zmk(sh.Mre[ad], sh.Mim[ad]), pij), zmk(cv, sv)).x;
-        dxx = fma(-gx * gx, wr, dxx);
-        dxy = fma(-gx * gy, wr, dxy);
-        dyy = fma(-gy * gy, wr, dyy);
+  // ---- phase 2 (not for a value-only evaluation, which needs log|det A| alone)
+  if (!VAL) {
+    const int j = col;
+    const double qx = sh.q[2 * j], qy = sh.q[2 * j + 1];
+    double* Fd = ws + (int64_t)w * L.per_walker + pick2(L.off_det, sp) + DS_N + 2 * s.nptot + 3 * m;
+    // H_b = S G'_b for output column tile ct (particles j), row tiles 2 rh + t:
+    // h[t][b][0] real, h[t][b][1] imaginary.  Beside it the wavefronts of row
+    // half 0 form diag(F_a F_b)_j = diag(A^-1 G_a G_b A)_j
+    // = sum_k' S(p_j, k') (-g^a g^b)_{p_k'} A(p_k', j) / piv_j from the same
+    // A(p_k', j) values (lane (lr, lc) summing k' = lr mod 4 for particle j, the
+    // four partial sums combined after the loop), O(m^2) in all.  The sin/cos
+    // constants stay in scalar registers here, where the accumulators take the
+    // vector ones.
+    const bool dg = rh == 0;
+    int prj = 0, swj = 0;
+    zd pij = zmk(0.0, 0.0);
+    if (dg && j < m) {
+      prj = sh.prow[j];
+      swj = gm_swz(prj);
+      pij = sh.pinv[j];
+    }
+    double dxx = 0.0, dxy = 0.0, dyy = 0.0;
+    gm_d4 h[2][2][2];
+  #pragma unroll
+    for (int t = 0; t < 2; t++)
+  #pragma unroll
+      for (int b = 0; b < 2; b++) h[t][b][0] = h[t][b][1] = gm_d4{0.0, 0.0, 0.0, 0.0};
+    // a wavefront with no live tile has neither outputs nor diagonal terms
+    // (its particles are >= m)
+    for (int ks = 0; live1 && 4 * ks < m; ks++) {
+      const int kp = 4 * ks + lr;
+      double bx_re = 0.0, bx_im = 0.0, by_re = 0.0, by_im = 0.0;
+      if (kp < m && j < m) {
+        const int o = sh.prow[kp];
+        const double gx = sh.g[2 * o], gy = sh.g[2 * o + 1];
+        double sv, cv;
+        qsincos_sreg(gx * qx + gy * qy, &sv, &cv);
+        bx_re = -gx * sv;
+        bx_im = gx * cv;
+        by_re = -gy * sv;
+        by_im = gy * cv;
+        if (dg) {
+          const int ad = prj * 64 + (kp ^ swj);
+          const double wr = zmul(zmul(zmk(sh.Mre[ad], sh.Mim[ad]), pij), zmk(cv, sv)).x;
+          dxx = fma(-gx * gx, wr, dxx);
+          dxy = fma(-gx * gy, wr, dxy);
+          dyy = fma(-gy * gy, wr, dyy);
+        }
+      }
+      zd a[2];
+  #pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const int r = 16 * (2 * rh + t) + lc;
+        const int ad = r * 64 + (kp ^ gm_swz(r));
+        a[t] = zmk(sh.Mre[ad], sh.Mim[ad]);
+      }
+  #pragma unroll
+      for (int t = 0; t < 2; t++) {
+        h[t][0][0] = gm_mfma(a[t].x, bx_re, h[t][0][0]);
+        h[t][0][1] = gm_mfma(a[t].x, bx_im, h[t][0][1]);
+        h[t][1][0] = gm_mfma(a[t].x, by_re, h[t][1][0]);
+        h[t][1][1] = gm_mfma(a[t].x, by_im, h[t][1][1]);
+      }
+  #pragma unroll
+      for (int t = 0; t < 2; t++) {
+        h[t][0][0] = gm_mfma(-a[t].y, bx_im, h[t][0][0]);
+        h[t][0][1] = gm_mfma(a[t].y, bx_re, h[t][0][1]);
+        h[t][1][0] = gm_mfma(-a[t].y, by_im, h[t][1][0]);
+        h[t][1][1] = gm_mfma(a[t].y, by_re, h[t][1][1]);
       }
     }
-    zd a[2];
-#pragma unroll
-    for (int t = 0; t < 2; t++) {
-      const int r = 16 * (2 * rh + t) + lc;
-      const int ad = r * 64 + (kp ^ gm_swz(r));
-      a[t] = zmk(sh.Mre[ad], sh.Mim[ad]);
+    if (dg) {
+  #pragma unroll
+      for (int o = 16; o < 64; o <<= 1) {
+        dxx += __shfl_xor(dxx, o, 64);
+        dxy += __shfl_xor(dxy, o, 64);
+        dyy += __shfl_xor(dyy, o, 64);
+      }
+      if (j < m && lr == 0) {
+        Fd[7 * j + 4] = dxx;
+        Fd[7 * j + 5] = dxy;
+        Fd[7 * j + 6] = dyy;
+      }
     }
-#pragma unroll
-    for (int t = 0; t < 2; t++) {
-      h[t][0][0] = gm_mfma(a[t].x, bx_re, h[t][0][0]);
-      h[t][0][1] = gm_mfma(a[t].x, bx_im, h[t][0][1]);
-      h[t][1][0] = gm_mfma(a[t].x, by_re, h[t][1][0]);
-      h[t][1][1] = gm_mfma(a[t].x, by_im, h[t][1][1]);
-    }
-#pragma unroll
-    for (int t = 0; t < 2; t++) {
-      h[t][0][0] = gm_mfma(-a[t].y, bx_im, h[t][0][0]);
-      h[t][0][1] = gm_mfma(a[t].y, bx_re, h[t][0][1]);
-      h[t][1][0] = gm_mfma(-a[t].y, by_im, h[t][1][0]);
-      h[t][1][1] = gm_mfma(a[t].y, by_re, h[t][1][1]);
-    }
-  }
-  if (dg) {
-#pragma unroll
-    for (int o = 16; o < 64; o <<= 1) {
-      dxx += __shfl_xor(dxx, o, 64);
-      dxy += __shfl_xor(dxy, o, 64);
-      dyy += __shfl_xor(dyy, o, 64);
-    }
-    if (j < m && lr == 0) {
-      Fd[7 * j + 4] = dxx;
-      Fd[7 * j + 5] = dxy;
-      Fd[7 * j + 6] = dyy;
-    }
-  }
-  // F_b(k, j) = H_b(p_k, j) / piv_k
-  zd* Fg = (zd*)(Fbuf + (int64_t)w * L.f_per_walker + pick2(L.f_off, sp));
-  if (j < m) {
-#pragma unroll
-    for (int t = 0; t < 2; t++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = 16 * (2 * rh + t) + lr + 4 * v;
-        if (r < m) {
-          const int k = sh.rank[r];
-          const zd pi = sh.pinv[k];
-#pragma unroll
-          for (int b = 0; b < 2; b++) {
-            const zd f = zmul(zmk(h[t][b][0][v], h[t][b][1][v]), pi);
-            Fg[(b * m + k) * m + j] = f;
-            // the diagonal F_b(j, j) also goes to the per-orbital terms
-            if (k == j) {
-              Fd[7 * j + 2 * b] = f.x;
-              Fd[7 * j + 2 * b + 1] = f.y;
+    // F_b(k, j) = H_b(p_k, j) / piv_k
+    zd* Fg = (zd*)(Fbuf + (int64_t)w * L.f_per_walker + pick2(L.f_off, sp));
+    if (j < m) {
+  #pragma unroll
+      for (int t = 0; t < 2; t++)
+  #pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int r = 16 * (2 * rh + t) + lr + 4 * v;
+          if (r < m) {
+            const int k = sh.rank[r];
+            const zd pi = sh.pinv[k];
+  #pragma unroll
+            for (int b = 0; b < 2; b++) {
+              const zd f = zmul(zmk(h[t][b][0][v], h[t][b][1][v]), pi);
+              Fg[(b * m + k) * m + j] = f;
+              // the diagonal F_b(j, j) also goes to the per-orbital terms
+              if (k == j) {
+                Fd[7 * j + 2 * b] = f.x;
+                Fd[7 * j + 2 * b + 1] = f.y;
+              }
             }
           }
         }
-      }
+    }
   }
   if (wave == 0) {
     double* dsc = ws + (int64_t)w * L.per_walker + pick2(L.off_det, sp);
@@ -388,14 +390,26 @@ __device__ __forceinline__ void gjm_run(DevSys s, WsLayout L, double* __restrict
 // every spin 49..64 particles: all 4 x 4 tiles take part
 __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm(DevSys s, WsLayout L, double* __restrict__ ws,
                                                        double* __restrict__ Fbuf) {
-  gjm_run<false>(s, L, ws, Fbuf);
+  gjm_run<false, false>(s, L, ws, Fbuf);
 }
 
 // a spin of 1..48 particles: the products of the tiles beyond m are skipped
 __global__ __launch_bounds__(GM_THREADS, 4) void k_gjm_part(DevSys s, WsLayout L,
                                                             double* __restrict__ ws,
                                                             double* __restrict__ Fbuf) {
-  gjm_run<true>(s, L, ws, Fbuf);
+  gjm_run<true, false>(s, L, ws, Fbuf);
+}
+
+// phase 1 and log|det A| only (the value of the wavefunction)
+__global__ __launch_bounds__(GM_THREADS, 4) void k_gjm_val(DevSys s, WsLayout L, double* __restrict__ ws,
+                                                           double* __restrict__ Fbuf) {
+  gjm_run<false, true>(s, L, ws, Fbuf);
+}
+
+__global__ __launch_bounds__(GM_THREADS, 4) void k_gjm_part_val(DevSys s, WsLayout L,
+                                                                double* __restrict__ ws,
+                                                                double* __restrict__ Fbuf) {
+  gjm_run<true, true>(s, L, ws, Fbuf);
 }
 
 size_t gjm_lds_bytes() { return sizeof(GmShared); }

@@ -99,6 +99,12 @@ struct qmc_ctx {
 int launch_eval(qmc_ctx* c, int nw, const double* d_x, double* d_props, double* d_g, double* d_h,
                 double* d_g2, int32_t* d_status, hipStream_t st, double delta,
                 int precision = QMC_PREC_FP64, const int* nact = nullptr);
+// log|psi| alone of nw configurations (d_ltf[w] = ltf, d_status[w] bit 2 =
+// singular): the pair kernel's Jastrow pass and the determinant's pivots,
+// without F, the drift, the Laplacian or the potential.  d_props receives
+// nothing used; the matrix-core determinant path only.
+int launch_value(qmc_ctx* c, int nw, const double* d_x, double* d_ltf, int32_t* d_status, double* d_props,
+                 hipStream_t st, int precision);
 void free_vmc(qmc_ctx* c);
 void free_vmc_graph(qmc_ctx* c);
 // Workspace of the evaluation kernels for nw walkers (allocates when it grows).

@@ -534,21 +534,24 @@ static int sem_check(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, i
     if (w->nw > 0 && !d->ainv) return arg_fail("null inverse buffer for single-electron moves");
     if (std::max(c->s.np[0], c->s.ntypes > 1 ? c->s.np[1] : 0) > QMC_MAX_ORB)
       return arg_fail("single-electron moves support at most 64 particles per spin", QMC_ERR_LIMIT);
-  } else if (w->nw > 0 && (!w->g || !w->h || !w->props || !w->grad2_ph)) {
-    return arg_fail("null walker buffer: moves with backflow keep the full evaluation");
+  } else if (w->nw > 0 && !w->props) {
+    return arg_fail("null walker buffer: moves with backflow keep ltf in the walkers' properties");
+  } else if (!c->use_gj) {
+    return arg_fail("moves with backflow need the matrix-core determinant path (QMC_NO_GJ is set)");
   }
   return QMC_OK;
 }
 
 // With backflow every quasi-coordinate depends on every electron, so one
 // electron's move changes every column of both orbital matrices and no
-// low-rank update applies: each move is a full evaluation of the proposed
-// configuration (the one the reference's compute_properties performs),
-// accepted with probability |psi'/psi|^2 = exp(-2 (ltf' - ltf)).  One
-// wavefront per walker copies the walker with electron j displaced (the
-// same three draws per move as the update path); after the evaluation one
-// wavefront per walker takes the decision and copies the proposal's
-// properties, drift and Laplacian sums into the walker when accepted.
+// low-rank update applies: each move evaluates the proposed configuration's
+// log|psi| (the pair kernel's Jastrow pass and quasi-coordinates, the
+// determinants' pivots: no F, drift, Laplacian or potential), accepted with
+// probability |psi'/psi|^2 = exp(-2 (ltf' - ltf)).  One wavefront per walker
+// copies the walker with electron j displaced (the same three draws per move
+// as the update path); after the evaluation one thread per walker takes the
+// decision and, when accepted, moves the electron and keeps the new ltf in
+// the walker's ltf slot.
 __global__ __launch_bounds__(256) void k_semf_propose(DevSys s, int nw, int j, double sigma,
                                                       const double* __restrict__ x, double* __restrict__ xn,
                                                       int64_t* __restrict__ rng, double* __restrict__ ua) {
@@ -574,31 +577,21 @@ __global__ __launch_bounds__(256) void k_semf_propose(DevSys s, int nw, int j, d
 }
 
 __global__ __launch_bounds__(256) void k_semf_accept(DevSys s, int nw, int j, int jltf, const double* __restrict__ xn,
-                                                     const double* __restrict__ pn, const double* __restrict__ gn,
-                                                     const double* __restrict__ hn, const int32_t* __restrict__ stn,
+                                                     const double* __restrict__ ltfn, const int32_t* __restrict__ stn,
                                                      const double* __restrict__ ua, double* __restrict__ x,
-                                                     double* __restrict__ props, double* __restrict__ g,
-                                                     double* __restrict__ h, double* __restrict__ dlogpsi,
+                                                     double* __restrict__ props, double* __restrict__ dlogpsi,
                                                      double* __restrict__ nacc) {
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int w = blockIdx.x * 4 + wv;
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nw) return;
   const int N = s.nptot, NP = s.n_props;
-  const double lo = props[(int64_t)w * NP + jltf], ln = pn[(int64_t)w * NP + jltf];
-  // a singular proposal (status bit 2) has psi' = 0; the underflow guard
-  // (bit 1) leaves ltf valid, and the decision reads ltf as metropolis_test
-  // does
+  const double lo = props[(int64_t)w * NP + jltf], ln = ltfn[w];
+  // a singular proposal (status bit 2) has psi' = 0
   if ((stn[w] & 2) != 0 || !(ua[w] < exp(-2.0 * (ln - lo)))) return;
-  for (int e = lane; e < NP; e += 64) props[(int64_t)w * NP + e] = pn[(int64_t)w * NP + e];
-  for (int e = lane; e < 2 * N; e += 64) g[(int64_t)w * 2 * N + e] = gn[(int64_t)w * 2 * N + e];
-  if (lane < 2) {
-    h[2 * w + lane] = hn[2 * w + lane];
-    x[(int64_t)w * 2 * N + 2 * j + lane] = xn[(int64_t)w * 2 * N + 2 * j + lane];
-  }
-  if (lane == 0) {
-    dlogpsi[w] += lo - ln;
-    nacc[w] += 1.0;
-  }
+  x[(int64_t)w * 2 * N + 2 * j] = xn[(int64_t)w * 2 * N + 2 * j];
+  x[(int64_t)w * 2 * N + 2 * j + 1] = xn[(int64_t)w * 2 * N + 2 * j + 1];
+  props[(int64_t)w * NP + jltf] = ln;
+  dlogpsi[w] += lo - ln;
+  nacc[w] += 1.0;
 }
 
 static int sem_moves_full(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, int j0, int j1, double sigma,
@@ -606,17 +599,14 @@ static int sem_moves_full(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state*
   const int nw = w->nw;
   int rc = ensure_vmc_scratch(c, nw);
   if (rc) return rc;
-  rc = ensure_ws(c, nw);
-  if (rc) return rc;
-  const dim3 grid((nw + 3) / 4);
   for (int j = j0; j < j1; j++) {
-    hipLaunchKernelGGL(k_semf_propose, grid, dim3(256), 0, st, c->s, nw, j, sigma, w->x, c->d_xn, w->rng,
-                       c->d_wold);
-    // the fixed-phase scratch is shared by proposals, as in the VMC step
-    rc = launch_eval(c, nw, c->d_xn, c->d_pn, c->d_gn, c->d_hn, w->grad2_ph, c->d_stn, st, c->delta, precision);
+    hipLaunchKernelGGL(k_semf_propose, dim3((nw + 3) / 4), dim3(256), 0, st, c->s, nw, j, sigma, w->x, c->d_xn,
+                       w->rng, c->d_wold);
+    // the proposal's ltf in d_hn (nw doubles of the proposal buffers)
+    rc = launch_value(c, nw, c->d_xn, c->d_hn, c->d_stn, c->d_pn, st, precision);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_semf_accept, grid, dim3(256), 0, st, c->s, nw, j, c->info.jltf - 1, c->d_xn, c->d_pn,
-                       c->d_gn, c->d_hn, c->d_stn, c->d_wold, w->x, w->props, w->g, w->h, d->dlogpsi, d->nacc);
+    hipLaunchKernelGGL(k_semf_accept, dim3((nw + 255) / 256), dim3(256), 0, st, c->s, nw, j, c->info.jltf - 1,
+                       c->d_xn, c->d_hn, c->d_stn, c->d_wold, w->x, w->props, d->dlogpsi, d->nacc);
   }
   HIPCHK(hipGetLastError());
   return QMC_OK;

@@ -388,10 +388,11 @@ int qmc_dmc_branch(qmc_ctx* ctx, int nw, double* d_wate, int64_t* d_rng2, int32_
  * electron changes one column of its spin's matrix A(k, j) = exp(i g_k . r_j)
  * and the inverse is updated (the two- and three-body Jastrow factors are
  * both supported).  With backflow every quasi-coordinate moves with the
- * electron, no low-rank update applies, and each move is a full evaluation
- * of the proposed configuration: there ainv is not used (may be NULL),
- * qmc_sem_refresh does nothing, and the walkers' props, g, h and grad2_ph
- * must hold the current evaluation (qmc_vmc_start) and are kept current.
+ * electron, no low-rank update applies, and each move evaluates the
+ * proposed configuration's log|psi| (Jastrow sums, quasi-coordinates and
+ * the determinants' pivots): there ainv is not used (may be NULL),
+ * qmc_sem_refresh does nothing, and the walkers' ltf slot of props must
+ * hold the current value (qmc_vmc_start) and is kept current.
  *
  * qmc_sem_state: device buffers owned by the caller, for w->nw walkers.
  * ainv holds per walker and spin the inverse, row = particle, column =
@@ -417,9 +418,9 @@ int qmc_sem_refresh(qmc_ctx* ctx, const qmc_walkers* w, const qmc_sem_state* d, 
  * acceptance), accepted with probability |R|^2 exp(-2 dU), where R is the
  * determinant ratio (A^-1 a')_j and dU the change of the Jastrow exponent
  * (the u2 sum and the three-body term of the full evaluation); with
- * backflow, with probability exp(-2 (ltf' - ltf)) from the evaluation of
- * the proposal.  Positions and inverses are updated in place; w->props,
- * w->g, w->h are not (without backflow) or are (with backflow).
+ * backflow, with probability exp(-2 (ltf' - ltf)) from the proposal's
+ * value.  Positions and inverses are updated in place; w->g, w->h and the
+ * other slots of w->props are stale until the next evaluation.
  */
 int qmc_sem_moves(qmc_ctx* ctx, const qmc_walkers* w, const qmc_sem_state* d, int j0, int j1,
                   double sigma, int precision, void* stream);
