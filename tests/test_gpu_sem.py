@@ -269,6 +269,38 @@ def test_sizes_across_row_classes(tmp_path, nup, ndown):
     assert err < 1e-8
 
 
+@pytest.mark.parametrize("twist", [(2, 3), (4, 1)])
+def test_twisted_boundary_conditions(tmp_path, twist):
+    """Twisted g-vectors (complex orbitals that are not periodic in the box,
+    positions never folded): the updated inverses stay exact and the summed
+    log|psi| change equals the full evaluations' at two twists of a 4x4
+    grid."""
+    print(f"\n[gpu] sem twist {twist}", flush=True)
+    import torch
+    hip = pkg("hip")
+    dg = pkg("deckgen")
+    dg.make_deck(str(tmp_path), "tw", 13, 13, 5.0, 5, nshell_k=12, u3=True, backflow="zero", jastrow="pade",
+                 ntwists=4)
+    d = pkg("deck").read_deck(str(tmp_path))
+    assert d.ntheta == 4
+    ctx = hip.Context(d, twist=twist)
+    x = dg.lattice_configs(16, d.nptot, d.el[0], 5)
+    wk = hip.Walkers(ctx, x, _states(16))
+    wk.start()
+    sm = hip.SingleMoves(wk)
+    jltf = ctx.info["jltf"] - 1
+    ltf0 = wk.props[:, jltf].clone()
+    sm.refresh()
+    for _ in range(4):
+        sm.sweep(0.7)
+    worst = _check_inverse_exact(d, ctx, wk, sm, 1e-9)
+    wk.start()
+    torch.cuda.synchronize()
+    err = float(np.max(np.abs(sm.dlogpsi.cpu().numpy() + (wk.props[:, jltf] - ltf0).cpu().numpy())))
+    print(f"   inverse rel err {worst:.1e}  |dlogpsi - full| {err:.1e}", flush=True)
+    assert err < 1e-9
+
+
 def test_moves_range_checks():
     """Electron ranges out of bounds and a step that is not positive are
     refused."""
