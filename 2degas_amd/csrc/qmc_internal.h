@@ -64,6 +64,8 @@ struct qmc_ctx {
   bool use_gj = false;   // matrix-core Gauss-Jordan determinant path (else k_det)
   bool b_zero = true;    // every backflow table is zero (quasi-coordinates = positions)
   DevSys* d_sys = nullptr;  // device copy of s for kernels that read it from memory
+  DevSys sys_sent{};        // what d_sys holds (valid when sys_valid)
+  bool sys_valid = false;
   bool gjm_part = false; // a spin below 49 particles: k_gjm_part, which skips the tiles beyond m
   size_t lds_gjm = 0;
   size_t lds_fgrad = 0;

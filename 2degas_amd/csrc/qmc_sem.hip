@@ -21,6 +21,8 @@
 // particle, column = orbital), in FP64 or, with QMC_PREC_MIXED, in FP32 (the
 // ratio and the update in FP32, the Jastrow and positions in FP64), and are
 // recomputed in FP64 by qmc_sem_refresh (Gauss-Jordan with partial pivoting).
+#include <cstring>
+
 #include "qmc_internal.h"
 
 // exp and log for the move loop, their constants kept in scalar registers
@@ -491,11 +493,16 @@ __global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves_
                                   (Z*)jb.ainv, jb.mm, jb.dlogpsi, jb.nacc);
 }
 
-// The context's system description in device memory, refreshed on the
-// stream before each launch that reads it (the g-vectors or the time step
-// may have changed since the last call).
+// The context's system description in device memory, sent again before a
+// launch that reads it only when it has changed since it was last sent (new
+// g-vectors, time step or tuning): a blocking copy, after the work already
+// queued on the stream that may still read the old one.
 static int upload_sys(qmc_ctx* c, hipStream_t st) {
-  HIPCHK(hipMemcpyAsync(c->d_sys, &c->s, sizeof(DevSys), hipMemcpyHostToDevice, st));
+  if (c->sys_valid && memcmp(&c->sys_sent, &c->s, sizeof(DevSys)) == 0) return QMC_OK;
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpy(c->d_sys, &c->s, sizeof(DevSys), hipMemcpyHostToDevice));
+  c->sys_sent = c->s;
+  c->sys_valid = true;
   return QMC_OK;
 }
 
