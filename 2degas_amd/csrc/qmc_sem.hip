@@ -1,11 +1,12 @@
-// qmc_sem.hip -- single-electron VMC moves with Sherman-Morrison updates of
-// the inverse Slater matrices, for trial functions without backflow (every
-// backflow table zero, so quasi-coordinates are positions), with the two-
-// and three-body Jastrow factors.  The reference has no such move: its move displaces
-// every electron and recomputes everything (reptation.f:4617-4680); this is
-// the single-electron scheme of the plain Slater-Jastrow function, checked
-// against the full evaluation (the log|psi| change of every accepted move)
-// and statistically against the all-electron walk.
+// qmc_sem.hip -- single-electron VMC moves.  For trial functions without
+// backflow (every backflow table zero, so quasi-coordinates are positions),
+// with the two- and three-body Jastrow factors: Sherman-Morrison updates of
+// the inverse Slater matrices.  With backflow: each move evaluates the
+// proposal's log|psi| (further below).  The reference has no such move: its
+// move displaces every electron and recomputes everything
+// (reptation.f:4617-4680).  Checked against the full evaluation (the log|psi|
+// change of every accepted move) and statistically against the all-electron
+// walk.
 //
 // Moving electron j of spin s from r to r' changes column j' (its index in
 // the spin) of A(k, j) = exp(i g_k . r_j) to a' and nothing else, so the
@@ -466,8 +467,8 @@ __global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves(
 // Several jobs (contexts of different N, decks and walker sets) in one
 // launch: job k owns blocks w0[k] .. w0[k] + nw[k] - 1, one per walker; each
 // block reads its job's system description from device memory (uniform
-// loads) and runs the same per-walker body.  MR is the largest per-spin
-// count of the jobs; a job with fewer particles leaves the extra rows idle.
+// loads) and runs the same per-walker body.  MR is the launch's row class
+// (16, 32 or 64); a job with fewer particles leaves the extra rows idle.
 struct SemJob {
   const DevSys* sys;
   double* x;
