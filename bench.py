@@ -128,6 +128,8 @@ def main():
     ap.add_argument("--rs", type=float, default=10.0)
     ap.add_argument("--shells", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the single-electron and served-jobs legs of the default line")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the size-based phase chunk of the pair kernels instead of qmc_autotune's")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -267,11 +269,68 @@ def main():
         "mixed": mixed,
         "autotune": tune,
     }
+    ctx.close()
+    if not args.no_extras:
+        out["single_electron"] = single_electron_leg(hip, deck, dg)
+        out["serve"] = serve_leg(args)
     if not args.no_cpu:
         cb = cpu_baseline(deckdir, x, args.cpu_seconds)
         out["cpu_baseline"] = cb
-    ctx.close()
     print(json.dumps(out))
+
+
+def single_electron_leg(hip, deck, dg):
+    """Beside the evaluation rate: single-electron sweeps at N=122 (61+61,
+    rs=10, Pade u2, zero backflow, 200 k-shells, 4096 walkers), FP64 and
+    with FP32 inverses, the FP64 refresh, and one full evaluation of the
+    same walkers, event-timed."""
+    import torch
+    abi = importlib.import_module("2degas_amd.abi")
+    tmp = tempfile.mkdtemp(prefix="degas_sel_")
+    dg.make_deck(os.path.join(tmp, "deck"), "n122", 61, 61, 10.0, 200, u3=False, backflow="zero", jastrow="pade")
+    d = deck.read_deck(os.path.join(tmp, "deck"))
+    nw = 4096
+    ctx = hip.Context(d, twist=(1, 1), device=0)
+    x = dg.lattice_configs(nw, d.nptot, d.el[0], 99)
+    sts = [abi.limbs_to_state([3535, 3467, 3262, (2 * (1549 + k) + 1) % 4096]) for k in range(nw)]
+    wk = hip.Walkers(ctx, x, sts)
+    wk.start()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(n):
+            fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / n
+
+    res = {"workload": "N=122 (61+61) rs=10, Pade u2, zero backflow, 200 k-shells, 4096 walkers, step 0.6"}
+    for prec, name in ((abi.QMC_PREC_FP64, "fp64"), (abi.QMC_PREC_MIXED, "fp32_inverse")):
+        sm = hip.SingleMoves(wk, precision=prec)
+        sm.refresh()
+        ms = timed(lambda: sm.sweep(0.6), 10)
+        res[name] = {"sweep_ms": ms, "electron_moves_per_s": 1e3 / ms * nw * d.nptot,
+                     "refresh_ms": timed(sm.refresh, 3)}
+        sm = None
+    res["full_evaluation_ms"] = timed(wk.start, 5)
+    wk = None
+    ctx.close()
+    return res
+
+
+def serve_leg(args):
+    """The served-jobs half of the metric: 40 VMC jobs with Poisson
+    arrivals at 20/s (the --serve workload), p50/p99 time to the error bar."""
+    sv = importlib.import_module("2degas_amd.serve")
+    specs = sv.poisson_jobs(40, 20.0, seed=2024, nw=256, target=2e-4, max_blocks=1000, steps_per_block=20)
+    _, summary = sv.serve(specs)
+    return {"workload": "40 VMC jobs, Poisson 20/s, N in 26/58/74/122 or polarized 37/49/61, rs U[1,20], "
+                        "Pade u2 + n26 backflow +-10%, 256 walkers, target 1e-4 Ha/electron",
+            **{k: summary[k] for k in ("p50_latency_s", "p99_latency_s", "jobs_per_s", "converged", "capped",
+                                       "wall_s")}}
 
 
 def bench_mixed(args, ctx, xt, props, g, h, g2, st, stream, ref64, value64):
