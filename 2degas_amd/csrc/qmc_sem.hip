@@ -162,26 +162,29 @@ __global__ __launch_bounds__(MR* CG) void k_sem_inverse(DevSys s, int sp, const 
           bm = cbest[k];
           p = cidx[k];
         }
+      // the pivot row's threads share u_k = T(p, k) / piv (k != c) and, in
+      // column c, 1 / piv; every thread then updates its entries from them
       if (l == p) {
+        const double dd = 1.0 / (f.x * f.x + f.y * f.y);
+        const double2 ip = make_double2(f.x * dd, -f.y * dd);
 #pragma unroll
-        for (int ii = 0; ii < EPT; ii++) prow[cg + CG * ii] = b[ii];
+        for (int ii = 0; ii < EPT; ii++) {
+          const int k = cg + CG * ii;
+          const double2 q = b[ii];
+          prow[k] = k == c ? ip : make_double2(q.x * ip.x - q.y * ip.y, q.x * ip.y + q.y * ip.x);
+        }
         used = true;
         if (cg == 0) rk[p] = c;
       }
       if (tid == 0) pr[c] = p;
       __syncthreads();
-      const double2 pv = prow[c];
-      const double dd = 1.0 / (pv.x * pv.x + pv.y * pv.y);
-      const double2 ip = make_double2(pv.x * dd, -pv.y * dd);
-      const double2 fi = make_double2(f.x * ip.x - f.y * ip.y, f.x * ip.y + f.y * ip.x);
 #pragma unroll
       for (int ii = 0; ii < EPT; ii++) {
         const int k = cg + CG * ii;
-        const double2 q = prow[k];
-        const double2 u = make_double2(q.x * ip.x - q.y * ip.y, q.x * ip.y + q.y * ip.x);
-        if (l == p) b[ii] = k == c ? ip : make_double2(-u.x, -u.y);
-        else if (k == c) b[ii] = fi;
-        else b[ii] = make_double2(b[ii].x - (f.x * u.x - f.y * u.y), b[ii].y - (f.x * u.y + f.y * u.x));
+        const double2 u = prow[k];
+        if (l == p) b[ii] = k == c ? u : make_double2(-u.x, -u.y);
+        else if (k == c) b[ii] = make_double2(f.x * u.x - f.y * u.y, f.x * u.y + f.y * u.x);
+        else b[ii] = make_double2(fma(-f.x, u.x, fma(f.y, u.y, b[ii].x)), fma(-f.x, u.y, fma(-f.y, u.x, b[ii].y)));
       }
     }
   }

@@ -399,7 +399,7 @@ class SingleMoves:
         live = new.abs().amax(dim=(2, 3, 4)) > 0
         return float((num / den)[live].max().item())
 
-    def tune_refresh(self, sigma, candidates=(16, 64, 256), tol=1e-4, cycles=3, stream=None):
+    def tune_refresh(self, sigma, candidates=(16, 64, 256, 1024), tol=1e-4, cycles=3, stream=None):
         """Refresh interval K (moves per walker between FP64 refreshes): for
         each candidate, `cycles` rounds of a refresh, K moves and a measured
         refresh (drift); returns the largest K whose worst drift stays
