@@ -234,7 +234,8 @@ static int sem_inverse_spin(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_stat
 // The FP64 inverse of a 64-row spin takes 64 registers per thread; left to
 // itself the compiler spends some 110 more and fits 2 waves per SIMD.  A
 // bound of 3 costs a few spilled registers and is 17% faster at N=122
-// (measured, 4096 walkers).  Other shapes fit 4 or more waves unbounded.
+// (measured, 4096 walkers).  Other shapes fit 4 or more waves unbounded (a
+// bound of 5 on the FP32 64-row shape spills and is 8% slower).
 template <typename Z, int MR>
 struct SemWaves {
   static constexpr int value = (sizeof(Z) == 16 && MR == 64) ? 3 : 1;
