@@ -328,11 +328,24 @@ def dmc_branch(ctx, wate, rng2, src, counts, stream=None):
         raise QmcError(rc, "qmc_dmc_branch")
 
 
+def backflow_zero(deck):
+    """True when every backflow table the deck's type pairs use is zero (the
+    quasi-coordinates are the positions), as the library finds it."""
+    nt = int(deck.ntypes)
+    for it in range(nt):
+        for jt in range(it, nt):
+            k = int(deck.iub[it * 2 + jt])
+            if k > 0 and np.any(deck.tables[k - 1].coef != 0.0):
+                return False
+    return True
+
+
 class SingleMoves:
     """Single-electron moves over a Walkers set (qmc_sem_state): the inverse
     Slater matrices ([nw][2][mm][mm] complex, float64 or, with
-    precision=abi.QMC_PREC_MIXED, float32), the accumulated log|psi| change and the
-    accepted-move count per walker."""
+    precision=abi.QMC_PREC_MIXED, float32; none for a deck with backflow,
+    whose moves evaluate each proposal), the accumulated log|psi| change and
+    the accepted-move count per walker."""
 
     def __init__(self, walkers, precision=0):
         import torch
@@ -341,9 +354,11 @@ class SingleMoves:
         ctx = walkers.ctx
         np_ = [int(v) for v in ctx.deck.np[:ctx.deck.ntypes]]
         self.mm = max(np_)
+        self.updates = backflow_zero(ctx.deck)
         nw, dev = walkers.nw, walkers.x.device
         rt = torch.float32 if self.precision == abi.QMC_PREC_MIXED else torch.float64
-        self.ainv = torch.zeros((nw, 2, self.mm, self.mm, 2), dtype=rt, device=dev)
+        shape = (nw, 2, self.mm, self.mm, 2) if self.updates else (1, 2)
+        self.ainv = torch.zeros(shape, dtype=rt, device=dev)
         self.dlogpsi = torch.zeros(nw, dtype=torch.float64, device=dev)
         self.nacc = torch.zeros(nw, dtype=torch.float64, device=dev)
         s = abi.QmcSemState()
@@ -391,6 +406,8 @@ class SingleMoves:
         (the carried ones are then replaced by the fresh ones: this is a
         refresh that also measures)."""
         import torch
+        if not self.updates:
+            return 0.0
         old = self.ainv.to(torch.float64, copy=True)
         self.refresh()
         new = self.ainv.to(torch.float64)
