@@ -65,6 +65,9 @@ struct qmc_ctx {
   bool b_zero = true;    // every backflow table is zero (quasi-coordinates = positions)
   DevSys* d_sys = nullptr;  // device copy of s for kernels that read it from memory
   DevSys sys_sent{};        // what d_sys holds (valid when sys_valid)
+  // single-electron move kernels: threads per inverse row, per precision
+  // (FP64, FP32) and row class (16, 32, 64 rows)
+  int sem_cg[2][3] = {{4, 2, 4}, {4, 2, 4}};
   bool sys_valid = false;
   bool gjm_part = false; // a spin below 49 particles: k_gjm_part, which skips the tiles beyond m
   size_t lds_gjm = 0;

@@ -236,9 +236,9 @@ static int sem_inverse_spin(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_stat
 // bound of 3 costs a few spilled registers and is 17% faster at N=122
 // (measured, 4096 walkers).  Other shapes fit 4 or more waves unbounded (a
 // bound of 5 on the FP32 64-row shape spills and is 8% slower).
-template <typename Z, int MR>
+template <typename Z, int MR, int CG>
 struct SemWaves {
-  static constexpr int value = (sizeof(Z) == 16 && MR == 64) ? 3 : 1;
+  static constexpr int value = (sizeof(Z) == 16 && MR == 64 && CG == 4) ? 3 : 1;
 };
 
 template <typename Z, int MR, int CG, bool U3>
@@ -460,7 +460,7 @@ __device__ __forceinline__ void sem_moves_walker(const DevSys& s, int w, int j0,
 }
 
 template <typename Z, int MR, int CG, bool U3>
-__global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves(const DevSys* __restrict__ sys,
+__global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR, CG>::value)) void k_sem_moves(const DevSys* __restrict__ sys,
                                                                                  int j0, int j1, int sp,
                                                                                  double sigma, double* x, int64_t* rng,
                                                                                  Z* ainv, int mm, double* dlogpsi,
@@ -490,7 +490,7 @@ struct SemJobs {
 };
 
 template <typename Z, int MR, int CG, bool U3>
-__global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR>::value)) void k_sem_moves_packed(const SemJobs jobs) {
+__global__ __launch_bounds__(MR* CG, (SemWaves<Z, MR, CG>::value)) void k_sem_moves_packed(const SemJobs jobs) {
   int k = 0;
   while (k + 1 < jobs.njobs && (int)blockIdx.x >= jobs.job[k + 1].w0) k++;
   const SemJob& jb = jobs.job[k];
@@ -511,10 +511,9 @@ static int upload_sys(qmc_ctx* c, hipStream_t st) {
   return QMC_OK;
 }
 
-template <typename Z, int MR>
+template <typename Z, int MR, int CG>
 static int launch_sem_moves(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, int j0, int j1, int sp,
                             double sigma, int mm, hipStream_t st) {
-  constexpr int CG = 4;
   const bool u3 = c->s.has_u3 != 0;
   const size_t lds = (u3 ? 4 : 2) * (size_t)c->s.nptot * sizeof(double);
   if (u3)
@@ -527,13 +526,32 @@ static int launch_sem_moves(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_stat
   return QMC_OK;
 }
 
+static int sem_class(int m) { return m <= 16 ? 0 : m <= 32 ? 1 : 2; }
+
+// The row class of the spin and the context's threads per row for it (16
+// rows: 4 threads, one wavefront; 32 and 64 rows: 2 or 4).
 template <typename Z>
 static int sem_moves_spin(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, int j0, int j1, int sp,
                           double sigma, int mm, hipStream_t st) {
-  const int m = c->s.np[sp];
-  if (m <= 16) return launch_sem_moves<Z, 16>(c, w, d, j0, j1, sp, sigma, mm, st);
-  if (m <= 32) return launch_sem_moves<Z, 32>(c, w, d, j0, j1, sp, sigma, mm, st);
-  return launch_sem_moves<Z, 64>(c, w, d, j0, j1, sp, sigma, mm, st);
+  const int cls = sem_class(c->s.np[sp]);
+  const int cg = c->sem_cg[sizeof(Z) == 16 ? 0 : 1][cls];
+  if (cls == 0) return launch_sem_moves<Z, 16, 4>(c, w, d, j0, j1, sp, sigma, mm, st);
+  if (cls == 1)
+    return cg == 2 ? launch_sem_moves<Z, 32, 2>(c, w, d, j0, j1, sp, sigma, mm, st)
+                   : launch_sem_moves<Z, 32, 4>(c, w, d, j0, j1, sp, sigma, mm, st);
+  return cg == 2 ? launch_sem_moves<Z, 64, 2>(c, w, d, j0, j1, sp, sigma, mm, st)
+                 : launch_sem_moves<Z, 64, 4>(c, w, d, j0, j1, sp, sigma, mm, st);
+}
+
+extern "C" int qmc_sem_set_tile(qmc_ctx* c, int precision, int rows, int threads_per_row) {
+  if (!c) return arg_fail("null context");
+  if (precision != QMC_PREC_FP64 && precision != QMC_PREC_MIXED) return arg_fail("unknown precision mode");
+  const int cls = rows == 16 ? 0 : rows == 32 ? 1 : rows == 64 ? 2 : -1;
+  if (cls < 0) return arg_fail("row class must be 16, 32 or 64");
+  if (threads_per_row != 4 && (threads_per_row != 2 || cls == 0))
+    return arg_fail("threads per row must be 4, or 2 for the 32- and 64-row classes");
+  c->sem_cg[precision == QMC_PREC_MIXED ? 1 : 0][cls] = threads_per_row;
+  return QMC_OK;
 }
 
 static int sem_check(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_state* d, int precision) {
@@ -665,18 +683,25 @@ extern "C" int qmc_sem_moves(qmc_ctx* c, const qmc_walkers* w, const qmc_sem_sta
   return QMC_OK;
 }
 
-template <typename Z, int MR, bool U3>
+template <typename Z, int MR, int CG, bool U3>
 static int launch_sem_packed(const SemJobs& jb, int nblocks, size_t lds, hipStream_t st) {
-  hipLaunchKernelGGL((k_sem_moves_packed<Z, MR, 4, U3>), dim3(nblocks), dim3(MR * 4), lds, st, jb);
+  hipLaunchKernelGGL((k_sem_moves_packed<Z, MR, CG, U3>), dim3(nblocks), dim3(MR * CG), lds, st, jb);
   HIPCHK(hipGetLastError());
   return QMC_OK;
 }
 
+template <typename Z, int MR, int CG>
+static int sem_packed_u3(const SemJobs& jb, int nblocks, bool u3, size_t lds, hipStream_t st) {
+  return u3 ? launch_sem_packed<Z, MR, CG, true>(jb, nblocks, lds, st)
+            : launch_sem_packed<Z, MR, CG, false>(jb, nblocks, lds, st);
+}
+
 template <typename Z>
-static int sem_packed_phase(const SemJobs& jb, int nblocks, int mr, bool u3, size_t lds, hipStream_t st) {
-  if (mr <= 16) return u3 ? launch_sem_packed<Z, 16, true>(jb, nblocks, lds, st) : launch_sem_packed<Z, 16, false>(jb, nblocks, lds, st);
-  if (mr <= 32) return u3 ? launch_sem_packed<Z, 32, true>(jb, nblocks, lds, st) : launch_sem_packed<Z, 32, false>(jb, nblocks, lds, st);
-  return u3 ? launch_sem_packed<Z, 64, true>(jb, nblocks, lds, st) : launch_sem_packed<Z, 64, false>(jb, nblocks, lds, st);
+static int sem_packed_phase(const SemJobs& jb, int nblocks, int cls, int cg, bool u3, size_t lds, hipStream_t st) {
+  if (cls == 0) return sem_packed_u3<Z, 16, 4>(jb, nblocks, u3, lds, st);
+  if (cls == 1)
+    return cg == 2 ? sem_packed_u3<Z, 32, 2>(jb, nblocks, u3, lds, st) : sem_packed_u3<Z, 32, 4>(jb, nblocks, u3, lds, st);
+  return cg == 2 ? sem_packed_u3<Z, 64, 2>(jb, nblocks, u3, lds, st) : sem_packed_u3<Z, 64, 4>(jb, nblocks, u3, lds, st);
 }
 
 extern "C" int qmc_sem_moves_packed(const qmc_sem_job* jobs, int njobs, int precision, void* stream) {
@@ -698,13 +723,15 @@ extern "C" int qmc_sem_moves_packed(const qmc_sem_job* jobs, int njobs, int prec
     if (rc) return rc;
   }
   // spin 0 of every job, then spin 1: each job sees its moves in the order
-  // qmc_sem_moves makes them.  Within a spin, one launch per row class
-  // (16, 32, 64 rows, as qmc_sem_moves picks), so that each job runs with
-  // the thread layout of its own call and its sums are added in the same
-  // order; the three-body variant also serves jobs without u3 (their u3
-  // terms are skipped and add nothing).
+  // qmc_sem_moves makes them.  Within a spin, one launch per row class and
+  // threads per row (as qmc_sem_moves picks for each job's context), so
+  // that each job runs with the thread layout of its own call and its sums
+  // are added in the same order; the three-body variant also serves jobs
+  // without u3 (their u3 terms are skipped and add nothing).
+  const int pi = precision == QMC_PREC_MIXED ? 1 : 0;
   for (int sp = 0; sp < 2; sp++)
-  for (int cls = 0; cls < 3; cls++) {
+  for (int cls = 0; cls < 3; cls++)
+  for (int cg = 2; cg <= 4; cg += 2) {
     SemJobs jb{};
     int nblocks = 0, mr = 0;
     bool u3 = false;
@@ -713,7 +740,7 @@ extern "C" int qmc_sem_moves_packed(const qmc_sem_job* jobs, int njobs, int prec
       qmc_ctx* c = jobs[k].ctx;
       if (sp >= c->s.ntypes || jobs[k].w->nw == 0) continue;
       const int m = c->s.np[sp];
-      if ((m <= 16 ? 0 : m <= 32 ? 1 : 2) != cls) continue;
+      if (sem_class(m) != cls || (cls == 0 ? 4 : c->sem_cg[pi][cls]) != cg) continue;
       const int a = std::max(jobs[k].j0, c->s.ipf[sp]), b = std::min(jobs[k].j1, c->s.ipf[sp] + c->s.np[sp]);
       if (a >= b) continue;
       SemJob& e = jb.job[jb.njobs++];
@@ -736,8 +763,9 @@ extern "C" int qmc_sem_moves_packed(const qmc_sem_job* jobs, int njobs, int prec
       lds = std::max(lds, (c->s.has_u3 ? 4 : 2) * (size_t)c->s.nptot * sizeof(double));
     }
     if (jb.njobs == 0) continue;
-    const int rc = precision == QMC_PREC_MIXED ? sem_packed_phase<float2>(jb, nblocks, mr, u3, lds, st)
-                                               : sem_packed_phase<double2>(jb, nblocks, mr, u3, lds, st);
+    (void)mr;
+    const int rc = precision == QMC_PREC_MIXED ? sem_packed_phase<float2>(jb, nblocks, cls, cg, u3, lds, st)
+                                               : sem_packed_phase<double2>(jb, nblocks, cls, cg, u3, lds, st);
     if (rc) return rc;
   }
   return QMC_OK;

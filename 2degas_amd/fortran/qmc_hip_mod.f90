@@ -208,6 +208,13 @@ module qmc_hip_mod
        real(c_double), intent(out) :: ms_out(2)
      end function qmc_autotune
 
+     integer(c_int) function qmc_sem_set_tile(ctx, precision, rows, threads_per_row) &
+          bind(C, name='qmc_sem_set_tile')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: precision, rows, threads_per_row
+     end function qmc_sem_set_tile
+
      integer(c_int) function qmc_rmc_start(ctx, r, prm, stream) bind(C, name='qmc_rmc_start')
        import :: c_int, c_ptr, qmc_reptiles, qmc_rmc_params
        type(c_ptr), value :: ctx, stream

@@ -20,7 +20,7 @@ EXPORTS = ("qmc_ctx_create", "qmc_ctx_destroy", "qmc_ctx_info", "qmc_set_twist",
            "qmc_vmc_start", "qmc_vmc_sweep", "qmc_vmc_status", "qmc_rmc_start",
            "qmc_rmc_sweep", "qmc_set_delta", "qmc_eval_batch_device_noalloc", "qmc_dmc_sweep",
            "qmc_dmc_branch", "qmc_sem_refresh", "qmc_sem_moves",
-           "qmc_sem_moves_packed", "qmc_autotune")
+           "qmc_sem_moves_packed", "qmc_autotune", "qmc_sem_set_tile")
 
 _lib = None
 
@@ -81,6 +81,7 @@ def lib():
                                     C.c_int, C.c_double, C.c_int, vp]
         L.qmc_sem_moves_packed.argtypes = [C.POINTER(abi.QmcSemJob), C.c_int, C.c_int, vp]
         L.qmc_autotune.argtypes = [vp, C.c_int, ip, dp, vp]
+        L.qmc_sem_set_tile.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -415,6 +416,46 @@ class SingleMoves:
         den = new.abs().amax(dim=(2, 3, 4)).clamp_min(1e-300)
         live = new.abs().amax(dim=(2, 3, 4)) > 0
         return float((num / den)[live].max().item())
+
+    def set_tile(self, rows, threads_per_row):
+        """qmc_sem_set_tile for this context and precision."""
+        rc = lib().qmc_sem_set_tile(self.wk.ctx.h, self.precision, int(rows), int(threads_per_row))
+        if rc:
+            raise QmcError(rc, "qmc_sem_set_tile")
+
+    def tune_tile(self, sigma, sweeps=3):
+        """Autotune the move kernel's tile: for the row classes of this
+        context's spins (32 or 64 rows), time `sweeps` sweeps with 2 and with
+        4 threads per row on a copy of the walkers (the walkers themselves
+        do not move) and keep the faster.  Returns {threads: ms per sweep}
+        and the choice (None when every spin is in the 16-row class, which
+        has one shape)."""
+        import torch
+        d = self.wk.ctx.deck
+        rows = sorted({16 if m <= 16 else 32 if m <= 32 else 64 for m in d.np[:d.ntypes] if m > 0} - {16})
+        if not rows or not self.updates:
+            return {}, None
+        wk = Walkers(self.wk.ctx, self.wk.x.cpu().numpy(), self.wk.rng.cpu().numpy())
+        wk.start()
+        sm = SingleMoves(wk, precision=self.precision)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        times = {}
+        for cg in (2, 4):
+            for r in rows:
+                sm.set_tile(r, cg)
+            sm.refresh()
+            sm.sweep(sigma)
+            torch.cuda.synchronize()
+            ev[0].record()
+            for _ in range(sweeps):
+                sm.sweep(sigma)
+            ev[1].record()
+            torch.cuda.synchronize()
+            times[cg] = ev[0].elapsed_time(ev[1]) / sweeps
+        best = min(times, key=times.get)
+        for r in rows:
+            self.set_tile(r, best)
+        return times, best
 
     def tune_refresh(self, sigma, candidates=(16, 64, 256, 1024), tol=1e-4, cycles=3, stream=None):
         """Refresh interval K (moves per walker between FP64 refreshes): for

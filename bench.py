@@ -540,6 +540,7 @@ def bench_sem(args, hip, deck, dg):
         nsw = max(args.steps, 5)
         for prec, name in ((abi.QMC_PREC_FP64, "fp64"), (abi.QMC_PREC_MIXED, "mixed")):
             sm = hip.SingleMoves(wk, precision=prec)
+            tile_ms, tile = sm.tune_tile(sigma)
             sm.refresh()
             for _ in range(max(args.warmup, 2)):
                 sm.sweep(sigma)
@@ -548,7 +549,8 @@ def bench_sem(args, hip, deck, dg):
             acc = float(sm.nacc.sum().item()) / (nw * d.nptot * (nsw + 1))
             r[name] = {"sweep_ms": ms, "sweeps_per_s": 1e3 / ms,
                        "electron_moves_per_s": 1e3 / ms * nw * d.nptot,
-                       "refresh_ms": timed(sm.refresh, 5), "acceptance": acc}
+                       "refresh_ms": timed(sm.refresh, 5), "acceptance": acc,
+                       "tile_threads_per_row": tile, "tile_ms": tile_ms}
             sm = None
         r["eval_ms"] = timed(wk.start, 5)
         delta = 0.05 * rs * rs / d.nptot
@@ -576,6 +578,7 @@ def bench_sem(args, hip, deck, dg):
         r = {"N": d.nptot, "walkers": nw, "rs": rs, "sigma": 0.5}
         for prec, name in ((abi.QMC_PREC_FP64, "fp64"), (abi.QMC_PREC_MIXED, "mixed")):
             sm = hip.SingleMoves(wk, precision=prec)
+            tile_ms, tile = sm.tune_tile(0.5)
             sm.refresh()
             for _ in range(max(args.warmup, 5)):
                 sm.sweep(0.5)
@@ -594,6 +597,7 @@ def bench_sem(args, hip, deck, dg):
             torch.cuda.synchronize()
             lerr = float((sm.dlogpsi + (wk.props[:, jltf] - ltf0)).abs().max().item())
             r[name] = {"sweep_ms": ms, "refresh_ms": rms, "acceptance": acc, "K": kbest,
+                       "tile_threads_per_row": tile, "tile_ms": tile_ms,
                        "drift_by_K": drifts, "logpsi_err_after_K": lerr,
                        "sweep_ms_with_refresh": ms + rms * d.nptot / kbest,
                        "electron_moves_per_s": 1e3 / ms * nw * d.nptot}

@@ -444,6 +444,17 @@ typedef struct qmc_sem_job {
 int qmc_sem_moves_packed(const qmc_sem_job* jobs, int njobs, int precision, void* stream);
 
 /*
+ * Tile shape of the single-electron move kernel for a context (no
+ * counterpart in the reference): threads per row of the inverse (4, or 2
+ * for the 32- and 64-row classes; a row class holds spins of up to 16, 32
+ * or 64 particles) at a precision.  Defaults: 2 for 32 rows, 4 otherwise.
+ * The walk depends on it only through the order of the Jastrow sums
+ * (rounding); an autotuner times the shapes (SingleMoves.tune_tile in the
+ * Python binding) and sets the faster.
+ */
+int qmc_sem_set_tile(qmc_ctx* ctx, int precision, int rows, int threads_per_row);
+
+/*
  * Per-kernel timing with HIP events recorded on the launch stream around each
  * kernel of every evaluation while enabled (no counterpart in the reference,
  * which has no timers).  qmc_timing(ctx, 1) resets and enables,

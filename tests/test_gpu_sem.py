@@ -222,6 +222,11 @@ def test_packed_jobs_equal_separate_calls(tmp_path, precision):
         return out
 
     sep, pak = make(), make()
+    # tiles other than the defaults on two of the contexts (the 32- and
+    # 64-row classes), the same in both runs
+    for run in (sep, pak):
+        run[2].set_tile(32, 4)
+        run[3].set_tile(64, 2)
     for _ in range(2):
         for k, sm in enumerate(sep):
             sm.moves(0, decks[k].nptot, sigmas[k])
@@ -299,6 +304,38 @@ def test_twisted_boundary_conditions(tmp_path, twist):
     err = float(np.max(np.abs(sm.dlogpsi.cpu().numpy() + (wk.props[:, jltf] - ltf0).cpu().numpy())))
     print(f"   inverse rel err {worst:.1e}  |dlogpsi - full| {err:.1e}", flush=True)
     assert err < 1e-9
+
+
+@pytest.mark.parametrize("nu,rows", [(29, 32), (61, 64)])
+@pytest.mark.parametrize("threads", [2, 4])
+def test_tile_shapes_keep_the_walk_exact(tmp_path, nu, rows, threads):
+    """Both tile shapes of the 32- and 64-row classes (2 or 4 threads per
+    inverse row): the updated inverses stay exact and the log|psi| sums
+    equal the full evaluations'; tune_tile picks one of the two and leaves
+    the walkers where they were."""
+    print(f"\n[gpu] sem tile {rows} rows x {threads} threads", flush=True)
+    import torch
+    dg = pkg("deckgen")
+    dg.make_deck(str(tmp_path), "tl", nu, nu, 5.0, 5, nshell_k=12, u3=True, backflow="zero", jastrow="pade")
+    d = pkg("deck").read_deck(str(tmp_path))
+    d, ctx, wk, sm = _setup(None, 16, deck=d)
+    x0 = wk.x.clone()
+    times, best = sm.tune_tile(0.5, sweeps=1)
+    torch.cuda.synchronize()
+    assert best in (2, 4) and set(times) == {2, 4}
+    assert torch.equal(wk.x, x0)
+    sm.set_tile(rows, threads)
+    jltf = ctx.info["jltf"] - 1
+    ltf0 = wk.props[:, jltf].clone()
+    sm.refresh()
+    for _ in range(2):
+        sm.sweep(0.5)
+    worst = _check_inverse_exact(d, ctx, wk, sm, 1e-8)
+    wk.start()
+    torch.cuda.synchronize()
+    err = float(np.max(np.abs(sm.dlogpsi.cpu().numpy() + (wk.props[:, jltf] - ltf0).cpu().numpy())))
+    print(f"   tuned {times} -> {best}; inverse rel err {worst:.1e}  |dlogpsi - full| {err:.1e}", flush=True)
+    assert err < 1e-8
 
 
 def test_moves_range_checks():
