@@ -249,7 +249,7 @@ __device__ __forceinline__ void sem_moves_walker(const DevSys& s, int w, int j0,
   typedef typename SemZ<Z>::R R;
   constexpr int T = MR * CG, EPT = MR / CG, NWV = T / 64 > 0 ? T / 64 : 1;
   constexpr int PMAX = (128 + T - 1) / T;  // partners per thread (N <= 128)
-  __shared__ Z ap[MR];    // new orbital column a'
+  __shared__ Z ap[2][MR];  // new orbital column a' (two buffers: moves alternate)
   __shared__ Z rowj[MR];  // row j' of the inverse before the update
   __shared__ Z rz_sh;     // R
   __shared__ double red[3 * NWV];
@@ -269,7 +269,7 @@ __device__ __forceinline__ void sem_moves_walker(const DevSys& s, int w, int j0,
     const int k = cg + CG * i;
     b[i] = (l < m && k < m) ? gin[l * mm + k] : Z{(R)0, (R)0};
   }
-  if (tid < MR) ap[tid] = Z{(R)0, (R)0};
+  if (tid < MR) ap[0][tid] = ap[1][tid] = Z{(R)0, (R)0};
   for (int e = tid; e < 2 * N; e += T) xs[e] = x[(int64_t)w * 2 * N + e];
   const uint64_t st0 = (uint64_t)rng[w];
   if (tid < nmv) {
@@ -306,28 +306,39 @@ __device__ __forceinline__ void sem_moves_walker(const DevSys& s, int w, int j0,
     }
   }
   __syncthreads();
+  // a'(k) = exp(i g_k . r') of a move into buffer ab, entry k by this thread
+  auto put_col = [&](Z* ab, int k, double px, double py) {
+    double so, co;
+    qsincos_sreg(s.gvec[2 * k] * px + s.gvec[2 * k + 1] * py, &so, &co);
+    ab[k] = Z{(R)co, (R)so};
+  };
+  // With a wavefront's worth of threads beyond the Jastrow sum's partners
+  // (the last m threads, all >= N), those threads form the next move's a'
+  // while the others sum, so that a move starts from a' already in LDS;
+  // otherwise threads 0..m-1 form it at the start of the move.  Either way
+  // from the same inputs (electron j+1 has not moved before its own move).
+  // FP32 inverses only: with FP64 ones the extra live values spill inside the
+  // loop, and the move measured 1% slower instead of 3% faster.
+  constexpr bool PRE = sizeof(typename SemZ<Z>::R) == 4 && PMAX == 1;
+  const bool pre = PRE && T - m >= N;
   for (int j = j0; j < j1; j++) {
-    const int jj = j - p0;
+    const int jj = j - p0, c = (j - j0) & 1;
     const double2 dx = step[j - j0];
     const double xo = xs[2 * j], yo = xs[2 * j + 1];
     const double xn = xo + dx.x, yn = yo + dx.y;
-    if (tid < m) {
-      double so, co;
-      qsincos_sreg(s.gvec[2 * tid] * xn + s.gvec[2 * tid + 1] * yn, &so, &co);
-      ap[tid] = Z{(R)co, (R)so};
-    }
+    if ((!pre || j == j0) && tid < m) put_col(ap[c], tid, xn, yn);
     __syncthreads();
     // t as explicit FMAs into two accumulator pairs (even and odd entries)
     R tr = 0, ti = 0, tr2 = 0, ti2 = 0;
 #pragma unroll
     for (int i = 0; i < EPT; i += 2) {
-      const Z a = ap[cg + CG * i];
+      const Z a = ap[c][cg + CG * i];
       tr = fma(b[i].x, a.x, tr);
       tr = fma(-b[i].y, a.y, tr);
       ti = fma(b[i].x, a.y, ti);
       ti = fma(b[i].y, a.x, ti);
       if (i + 1 < EPT) {
-        const Z a2 = ap[cg + CG * (i + 1)];
+        const Z a2 = ap[c][cg + CG * (i + 1)];
         tr2 = fma(b[i + 1].x, a2.x, tr2);
         tr2 = fma(-b[i + 1].y, a2.y, tr2);
         ti2 = fma(b[i + 1].x, a2.y, ti2);
@@ -382,6 +393,10 @@ __device__ __forceinline__ void sem_moves_walker(const DevSys& s, int w, int j0,
           gjy -= fn * pn.uy;
         }
       }
+    }
+    if (pre && j + 1 < j1 && tid >= T - m) {
+      const double2 d = step[j + 1 - j0];
+      put_col(ap[c ^ 1], tid - (T - m), xs[2 * (j + 1)] + d.x, xs[2 * (j + 1) + 1] + d.y);
     }
     du = wave_sum(du);
     if (U3) {
